@@ -1,0 +1,171 @@
+/* bhrt.h - C ABI of the MI355X (gfx950) Kerr tracer: the drop-in replacement for bhrt's GPU
+   Renderer on the per-pixel geodesic + thermal-synchrotron path.
+
+   Reference interface replaced (paths relative to the reference repository root):
+     class Renderer                       blackholes/src/renderer.hpp:72-88
+     SceneGPU / FrameGPU PODs             blackholes/src/scene.hpp:12-80
+     SceneTables / RenderSettings         blackholes/src/renderer.hpp:19-29
+     RenderStats                          blackholes/src/renderer.hpp:64-68
+     HdrFrame.px (W*H*4 floats)           blackholes/src/renderer.hpp:37-46
+
+   Conventions: plain C types only; integer status codes (BH_OK == 0), never abort; error text is
+   copied to the caller's buffer or read with bh_last_error(). One context per device; a context is
+   not reentrant (callers serialise). Every call is synchronous, like the reference's (render.cu:697,
+   713, 725, 770), except bh_render_resident, which leaves the frame in device memory. */
+#ifndef BHRT_H
+#define BHRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_OK 0
+#define BH_ERR_ARG (-1)
+#define BH_ERR_HIP (-2)
+#define BH_ERR_NODEVICE (-3)
+#define BH_ERR_UNSUPPORTED (-4)
+
+enum { BH_MODE_VISIBLE = 0, BH_MODE_RADIO = 1 };      /* scene.hpp:10 RenderMode */
+enum { BH_PREC_FP64 = 0, BH_PREC_MIXED = 1 };         /* plain FP64 (parity) or mixed FP32/FP64 */
+enum { BH_EMIS_EXACT = 0, BH_EMIS_MLP = 1 };          /* synchrotron coefficients: exact or surrogate */
+enum { BH_FATE_LIMIT = 0, BH_FATE_ESCAPED = 1, BH_FATE_CAPTURED = -1 };
+
+/* SceneGPU (scene.hpp:12-69): the scalars on the vacuum/radio path. The thin disk, sky and colour
+   fields are out of scope (visible-mode presentation) and absent. */
+typedef struct bh_scene {
+    double a, rh, r_isco;
+    double tol;             /* DP54 tolerance (render.tolerance) */
+    int max_steps;
+    double r_escape, r_stop, step_frac;
+    int mode;               /* BH_MODE_* ; visible = vacuum geodesics only */
+    int flow_on;
+    double n0, n_index, theta_index, theta0, h_r, beta_p, r_flow_max, nu_obs, rg_cm, flow_kep;
+    float flow_turb_amp;    /* > 0 enables the turbulence grid of bh_tables */
+    uint32_t seed;
+} bh_scene;
+
+/* SceneTables (renderer.hpp:19-23) plus the turbulence input. The K2 table is ln K2(1/Theta) on a
+   ln Theta grid (scene_build.cpp:159-170). The turbulence grid replaces the baked texture slices
+   (render.cu:448-460, emission.cuh:112-123): turb_nt time slices of a cell-centred turb_n^3
+   Cartesian grid over [-turb_half, turb_half]^3 (x fastest), holding the density factor. With
+   turb_nt > 1, slice k sits at coordinate time turb_t0 + k*turb_dt and the tracer interpolates
+   linearly in the emission time of each step (slow light). */
+typedef struct bh_tables {
+    const float* k2_ln;
+    int k2_n;
+    float k2_l0, k2_dl;
+    const float* turb;
+    int turb_n, turb_nt;
+    double turb_half, turb_t0, turb_dt;
+} bh_tables;
+
+/* FrameGPU (scene.hpp:71-80). kp = KerrPoint at the camera (kerr.hpp:20-25): r, sin th, cos th,
+   Sigma, Delta, A, omega, e_nu, e_psi, sqrt(Sigma), sqrt(Sigma Delta). */
+typedef struct bh_frame {
+    double r, th, ph, t;
+    double kp[11];
+    double F[3], R[3], U[3], beta[3];
+    double cx, cy, pix;
+    int W, H, spp;
+    uint32_t seed;
+} bh_frame;
+
+/* RenderSettings (renderer.hpp:25-29) */
+typedef struct bh_settings {
+    int W, H, spp, spp_max;
+    float target_err;
+} bh_settings;
+
+/* Per-call options (new; the reference has one fixed path). */
+typedef struct bh_trace_opts {
+    int precision;    /* BH_PREC_* */
+    int emissivity;   /* BH_EMIS_* (radio only) */
+    int jitter;       /* 0: pixel-centre samples; 1: R2 + PCG jitter as render.cu:70-85 */
+    int spp;          /* samples per pixel; 0 -> frame->spp */
+} bh_trace_opts;
+
+/* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */
+typedef struct bh_stats {
+    double ms_trace;          /* host wall time of the call */
+    double ms_kernel;         /* HIP-event time of the trace kernels on the context stream */
+    long long samples;
+    long long step_limited;   /* rays with fate 0 */
+    long long step_attempts;  /* DP54 steps tried (accepted + rejected) */
+    long long steps_accepted;
+    long long flow_steps;     /* accepted steps inside the flow (radio) */
+    long long fp64_steps;     /* mixed precision: attempts taken on the FP64 path */
+    double flux_sum;          /* sum over samples of I_nu (radio) */
+} bh_stats;
+
+/* Per-ray record of the tile interface (new: trace.cuh:62 keeps these internal). */
+typedef struct bh_ray_out {
+    double y[6];              /* r, dr/dtau, u = cos th, du/dtau, phi, t when the ray stopped */
+    double th_inf, ph_inf;    /* escape direction (trace.cuh:49-55), escaped rays only */
+    double I[4];              /* radio: I_nu, and its n = 1, 2, >= 3 parts */
+    double trans;
+    double psi;               /* swept angle (radio) */
+    int fate, ncross, order, steps, accepted, flow_steps;
+} bh_ray_out;
+
+typedef struct bh_ctx bh_ctx;
+
+/* int bh_device_count()                                   renderer.hpp:70, render.cu:491-495 */
+int bh_device_count(void);
+
+/* Renderer::init(device, scene, tables, sky, settings, err)   renderer.hpp:76-77, render.cu:531-656
+   The sky texture is out of scope. Tables are copied to device memory. */
+int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const bh_settings* settings,
+              bh_ctx** out, char* err, size_t errlen);
+
+/* Renderer::update_scene(scene): scalars only, tables stay    renderer.hpp:78, render.cu:658-675 */
+int bh_update_scene(bh_ctx* ctx, const bh_scene* scene);
+
+/* Current scene scalars of a context. */
+int bh_get_scene(const bh_ctx* ctx, bh_scene* scene);
+
+/* Renderer::render(frame, post, percentile, out, stats)      renderer.hpp:79, render.cu:755-795
+   Frame interface: traces W*H pixels of `frame` (spp samples each, plain mean; adaptive sampling
+   and post-processing are not part of this path) into px_out[W*H*4] = (I, I_n1, I_n2, I_n>=3) per
+   pixel, row-major from the top-left. px_out may be NULL. */
+int bh_render(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, float* px_out, bh_stats* stats);
+
+/* As bh_render, but the frame stays in device memory (float4 per pixel, returned in *d_px, owned by
+   the context and valid until the next call); only the counters come back. Used to time the tracer
+   with its inputs and outputs resident in HBM. */
+int bh_render_resident(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, const void** d_px,
+                       bh_stats* stats);
+
+/* Tile interface (k_pass1's per-pixel launch, render.cu:89-128, with every per-ray output): rays of
+   pixels x0 <= px < x0+w, y0 <= py < y0+h, one sample each (centre or jittered sample 0), written
+   row-major to out[w*h]. */
+int bh_trace_tile(bh_ctx* ctx, const bh_frame* frame, int x0, int y0, int w, int h, const bh_trace_opts* opts,
+                  bh_ray_out* out);
+
+/* Renderer::radio_flux(frame, n, pix_sr): flux density (Jy) of an n x n, spp = 2 jittered frame
+   renderer.hpp:81, render.cu:797-809 */
+int bh_radio_flux(bh_ctx* ctx, const bh_frame* frame, int n, double pix_sr, double* jy);
+
+/* vacuum_fate (kerr.hpp:184-203) for n photons; photons[9*i..] = (lam, eta, E, y[6]) as
+   camera_photon returns them. fates[i] in {+1, -1, 0}. Used by the critical-curve search
+   (shot.cpp:148-164) and the self-test. */
+int bh_vacuum_fates(bh_ctx* ctx, const double* photons, int n, double tol, double r_escape, int max_steps,
+                    int* fates);
+
+/* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context
+   needed; device 0). fn: 0 exp, 1 log, 2 expm1, 3 cbrt, 4 x^-0.2, 5 sin, 6 cos, 7 atan2(x, y),
+   8 acos, 9 sqrt, 10 x / y. Used by the tests that pin the device functions bit for bit. */
+int bh_math_eval(int fn, const double* x, const double* y, double* out, int n);
+
+/* Renderer::~Renderer                                           render.cu:526-529 */
+void bh_destroy(bh_ctx* ctx);
+
+/* Last error text of a context (or of the last failed bh_create when ctx is NULL). */
+const char* bh_last_error(const bh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

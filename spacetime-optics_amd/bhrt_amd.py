@@ -1,0 +1,346 @@
+"""ctypes binding of libbhrt.so (include/bhrt.h and include/bhrt_host.h).
+
+Plumbing for tests and the benchmark: every call goes to the HIP library; there is no Python or CPU
+fallback, and loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbhrt.so")
+
+BH_OK, BH_ERR_ARG, BH_ERR_HIP, BH_ERR_NODEVICE, BH_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
+MODE_VISIBLE, MODE_RADIO = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+EMIS_EXACT, EMIS_MLP = 0, 1
+
+# Every symbol declared in include/bhrt.h and include/bhrt_host.h
+EXPORTS = [
+    "bh_device_count", "bh_create", "bh_update_scene", "bh_get_scene", "bh_render", "bh_render_resident",
+    "bh_trace_tile", "bh_radio_flux", "bh_vacuum_fates", "bh_math_eval", "bh_destroy", "bh_last_error",
+    "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
+    "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
+]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("a", C.c_double), ("rh", C.c_double), ("r_isco", C.c_double), ("tol", C.c_double),
+        ("max_steps", C.c_int), ("r_escape", C.c_double), ("r_stop", C.c_double), ("step_frac", C.c_double),
+        ("mode", C.c_int), ("flow_on", C.c_int),
+        ("n0", C.c_double), ("n_index", C.c_double), ("theta_index", C.c_double), ("theta0", C.c_double),
+        ("h_r", C.c_double), ("beta_p", C.c_double), ("r_flow_max", C.c_double), ("nu_obs", C.c_double),
+        ("rg_cm", C.c_double), ("flow_kep", C.c_double), ("flow_turb_amp", C.c_float), ("seed", C.c_uint32),
+    ]
+
+
+class Tables(C.Structure):
+    _fields_ = [
+        ("k2_ln", C.POINTER(C.c_float)), ("k2_n", C.c_int), ("k2_l0", C.c_float), ("k2_dl", C.c_float),
+        ("turb", C.POINTER(C.c_float)), ("turb_n", C.c_int), ("turb_nt", C.c_int),
+        ("turb_half", C.c_double), ("turb_t0", C.c_double), ("turb_dt", C.c_double),
+    ]
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("r", C.c_double), ("th", C.c_double), ("ph", C.c_double), ("t", C.c_double),
+        ("kp", C.c_double * 11), ("F", C.c_double * 3), ("R", C.c_double * 3), ("U", C.c_double * 3),
+        ("beta", C.c_double * 3), ("cx", C.c_double), ("cy", C.c_double), ("pix", C.c_double),
+        ("W", C.c_int), ("H", C.c_int), ("spp", C.c_int), ("seed", C.c_uint32),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("spp", C.c_int), ("spp_max", C.c_int), ("target_err", C.c_float)]
+
+
+class TraceOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_trace", C.c_double), ("ms_kernel", C.c_double), ("samples", C.c_longlong),
+        ("step_limited", C.c_longlong), ("step_attempts", C.c_longlong), ("steps_accepted", C.c_longlong),
+        ("flow_steps", C.c_longlong), ("fp64_steps", C.c_longlong), ("flux_sum", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+RAY_DTYPE = np.dtype([
+    ("y", "<f8", (6,)), ("th_inf", "<f8"), ("ph_inf", "<f8"), ("I", "<f8", (4,)), ("trans", "<f8"),
+    ("psi", "<f8"), ("fate", "<i4"), ("ncross", "<i4"), ("order", "<i4"), ("steps", "<i4"),
+    ("accepted", "<i4"), ("flow_steps", "<i4"),
+])
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_lib = None
+
+
+def load():
+    """Loads libbhrt.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() or make -C spacetime-optics_amd")
+    L = C.CDLL(LIB_PATH)
+    L.bh_device_count.restype = C.c_int
+    L.bh_create.restype = C.c_int
+    L.bh_create.argtypes = [C.c_int, C.POINTER(Scene), C.POINTER(Tables), C.POINTER(Settings), C.POINTER(C.c_void_p),
+                            C.c_char_p, C.c_size_t]
+    L.bh_update_scene.restype = C.c_int
+    L.bh_update_scene.argtypes = [C.c_void_p, C.POINTER(Scene)]
+    L.bh_get_scene.restype = C.c_int
+    L.bh_get_scene.argtypes = [C.c_void_p, C.POINTER(Scene)]
+    L.bh_render.restype = C.c_int
+    L.bh_render.argtypes = [C.c_void_p, C.POINTER(Frame), C.POINTER(TraceOpts), _F, C.POINTER(Stats)]
+    L.bh_render_resident.restype = C.c_int
+    L.bh_render_resident.argtypes = [C.c_void_p, C.POINTER(Frame), C.POINTER(TraceOpts), C.POINTER(C.c_void_p),
+                                     C.POINTER(Stats)]
+    L.bh_trace_tile.restype = C.c_int
+    L.bh_trace_tile.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.POINTER(TraceOpts), C.c_void_p]
+    L.bh_radio_flux.restype = C.c_int
+    L.bh_radio_flux.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_double, _D]
+    L.bh_vacuum_fates.restype = C.c_int
+    L.bh_vacuum_fates.argtypes = [C.c_void_p, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.bh_math_eval.restype = C.c_int
+    L.bh_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
+    L.bh_destroy.argtypes = [C.c_void_p]
+    L.bh_last_error.restype = C.c_char_p
+    L.bh_last_error.argtypes = [C.c_void_p]
+    L.bhh_config_str.restype = C.c_int
+    L.bhh_config_str.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+    L.bhh_build_scene.restype = C.c_int
+    L.bhh_build_scene.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Scene), _F, _F, _F, _D, C.POINTER(TraceOpts),
+                                  C.c_char_p, C.c_size_t]
+    L.bhh_turbulence.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint32, _F]
+    L.bhh_build_shot.restype = C.c_int
+    L.bhh_build_shot.argtypes = [C.c_char_p, C.c_char_p, _D, C.c_int]
+    L.bhh_make_frame.argtypes = ([C.c_double] * 8 + [C.c_int] + [C.c_double] * 7 + [C.c_int] * 4 + [C.c_uint32,
+                                 C.POINTER(Frame)])
+    L.bhh_create_from_ini.restype = C.c_int
+    L.bhh_create_from_ini.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p),
+                                      C.POINTER(Scene), C.POINTER(Frame), C.POINTER(TraceOpts), _D, C.c_char_p,
+                                      C.c_size_t]
+    L.bhh_find_critical_point.restype = C.c_int
+    L.bhh_find_critical_point.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_double, _D]
+    L.bhh_selftest.restype = C.c_int
+    L.bhh_selftest.argtypes = [C.c_int, _D, C.c_char_p, C.c_size_t]
+    _lib = L
+    return L
+
+
+class BhError(RuntimeError):
+    pass
+
+
+def device_count():
+    return load().bh_device_count()
+
+
+# --- host layer (no GPU needed) ------------------------------------------------------------------
+def config_str(ini, key, sets=""):
+    buf = C.create_string_buffer(4096)
+    r = load().bhh_config_str(ini.encode(), sets.encode(), key.encode(), buf, 4096)
+    if r < 0:
+        raise BhError(buf.value.decode())
+    return buf.value.decode() if r == 1 else None
+
+
+def build_scene(ini, sets=""):
+    """(Scene, k2 table, (l0, dl), info dict, TraceOpts) of a shot .ini."""
+    S = Scene()
+    k2 = np.zeros(512, dtype=np.float32)
+    l0, dl = C.c_float(), C.c_float()
+    info = np.zeros(6)
+    opts = TraceOpts()
+    err = C.create_string_buffer(512)
+    rc = load().bhh_build_scene(ini.encode(), sets.encode(), C.byref(S), k2.ctypes.data_as(_F), C.byref(l0),
+                                C.byref(dl), info.ctypes.data_as(_D), C.byref(opts), err, 512)
+    if rc != BH_OK:
+        raise BhError(err.value.decode())
+    keys = ["mass_msun", "distance_cm", "rg_cm", "uas_per_m", "flux_target_jy", "normalize_fov_m"]
+    return S, k2, (l0.value, dl.value), dict(zip(keys, info)), opts
+
+
+def turbulence(n=128, nt=1, half=60.0, sigma=0.3, corr_m=5.0, corr_t=30.0, dt=10.0, seed=1):
+    out = np.zeros((nt, n, n, n), dtype=np.float32)
+    load().bhh_turbulence(n, nt, half, sigma, corr_m, corr_t, dt, seed, out.ctypes.data_as(_F))
+    return out
+
+
+def build_shot(ini, sets="", maxn=4096):
+    out = np.zeros((maxn, 14))
+    n = load().bhh_build_shot(ini.encode(), sets.encode(), out.ctypes.data_as(_D), maxn)
+    if n < 0:
+        raise BhError(f"build_shot failed for {ini}")
+    return out[:n]
+
+
+def make_frame(a, r, incl, phi=0.0, roll=0.0, yaw=0.0, pitch=0.0, fov=30.0, fov_in_m=True, zoom=1.0, target=0.0,
+               target_x=0.0, target_y=0.0, off_x=0.0, off_y=0.0, time_m=0.0, orbit=False, W=1, H=None, spp=1, seed=0):
+    F = Frame()
+    load().bhh_make_frame(a, r, incl, phi, roll, yaw, pitch, fov, int(fov_in_m), zoom, target, target_x, target_y,
+                          off_x, off_y, time_m, int(orbit), W, W if H is None else H, spp, seed, C.byref(F))
+    return F
+
+
+def k2_from_scene(ini, sets=""):
+    _, k2, (l0, dl), _, _ = build_scene(ini, sets)
+    return k2, l0, dl
+
+
+MATH_FN = {"exp": 0, "log": 1, "expm1": 2, "cbrt": 3, "pow_m02": 4, "sin": 5, "cos": 6, "atan2": 7, "acos": 8,
+           "sqrt": 9, "div": 10}
+
+
+def math_eval(fn, x, y=None):
+    """Device evaluation of one bh_math function (diagnostic entry bh_math_eval)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    out = np.zeros_like(x)
+    rc = load().bh_math_eval(MATH_FN[fn], x.ctypes.data_as(_D), y.ctypes.data_as(_D), out.ctypes.data_as(_D), len(x))
+    if rc != BH_OK:
+        raise BhError(f"bh_math_eval({fn}) failed ({rc})")
+    return out
+
+
+def selftest(device=0):
+    rows = np.zeros((8, 6))
+    err = C.create_string_buffer(512)
+    rc = load().bhh_selftest(device, rows.ctypes.data_as(_D), err, 512)
+    if rc != BH_OK:
+        raise BhError(err.value.decode())
+    return rows
+
+
+# --- device context ---------------------------------------------------------------------------------
+class Tracer:
+    """A device context (bh_ctx)."""
+
+    def __init__(self, scene, k2=None, k2_l0=0.0, k2_dl=0.0, grid=None, turb_half=60.0, turb_t0=0.0, turb_dt=0.0,
+                 device=0, settings=None):
+        L = load()
+        T = Tables()
+        self._keep = []
+        if k2 is not None:
+            k2 = np.ascontiguousarray(k2, dtype=np.float32)
+            self._keep.append(k2)
+            T.k2_ln = k2.ctypes.data_as(_F)
+            T.k2_n = len(k2)
+            T.k2_l0 = k2_l0
+            T.k2_dl = k2_dl
+        if grid is not None:
+            g = np.ascontiguousarray(grid, dtype=np.float32)
+            if g.ndim == 3:
+                g = g[None]
+            self._keep.append(g)
+            T.turb = g.ctypes.data_as(_F)
+            T.turb_n = g.shape[-1]
+            T.turb_nt = g.shape[0]
+            T.turb_half = turb_half
+            T.turb_t0 = turb_t0
+            T.turb_dt = turb_dt
+        rs = settings or Settings()
+        ctx = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.bh_create(device, C.byref(scene), C.byref(T), C.byref(rs), C.byref(ctx), err, 512)
+        if rc != BH_OK:
+            raise BhError(f"bh_create: {err.value.decode()} ({rc})")
+        self.ctx = ctx
+        self.L = L
+
+    @classmethod
+    def from_ini(cls, ini, sets="", device=0, normalize=True, frame_index=0):
+        """(Tracer, Scene, Frame, TraceOpts, flux Jy) for a shot .ini."""
+        L = load()
+        self = cls.__new__(cls)
+        self._keep = []
+        self.L = L
+        ctx = C.c_void_p()
+        S, F, O = Scene(), Frame(), TraceOpts()
+        jy = C.c_double()
+        err = C.create_string_buffer(512)
+        rc = L.bhh_create_from_ini(ini.encode(), sets.encode(), device, int(normalize), frame_index, C.byref(ctx),
+                                   C.byref(S), C.byref(F), C.byref(O), C.byref(jy), err, 512)
+        if rc != BH_OK:
+            raise BhError(f"bhh_create_from_ini: {err.value.decode()} ({rc})")
+        self.ctx = ctx
+        return self, S, F, O, jy.value
+
+    def _check(self, rc, what):
+        if rc != BH_OK:
+            raise BhError(f"{what}: {self.L.bh_last_error(self.ctx).decode()} ({rc})")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.bh_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update_scene(self, S):
+        self._check(self.L.bh_update_scene(self.ctx, C.byref(S)), "bh_update_scene")
+
+    def scene(self):
+        S = Scene()
+        self._check(self.L.bh_get_scene(self.ctx, C.byref(S)), "bh_get_scene")
+        return S
+
+    @staticmethod
+    def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1):
+        return TraceOpts(precision, emissivity, jitter, spp)
+
+    def render(self, F, opts=None):
+        px = np.zeros((F.H, F.W, 4), dtype=np.float32)
+        st = Stats()
+        o = opts or self.opts()
+        self._check(self.L.bh_render(self.ctx, C.byref(F), C.byref(o), px.ctypes.data_as(_F), C.byref(st)),
+                    "bh_render")
+        return px, st
+
+    def render_resident(self, F, opts=None):
+        st = Stats()
+        o = opts or self.opts()
+        ptr = C.c_void_p()
+        self._check(self.L.bh_render_resident(self.ctx, C.byref(F), C.byref(o), C.byref(ptr), C.byref(st)),
+                    "bh_render_resident")
+        return ptr.value, st
+
+    def trace_tile(self, F, x0, y0, w, h, opts=None):
+        out = np.zeros(w * h, dtype=RAY_DTYPE)
+        o = opts or self.opts()
+        self._check(self.L.bh_trace_tile(self.ctx, C.byref(F), x0, y0, w, h, C.byref(o), out.ctypes.data),
+                    "bh_trace_tile")
+        return out.reshape(h, w)
+
+    def radio_flux(self, F, n, pix_sr):
+        jy = C.c_double()
+        self._check(self.L.bh_radio_flux(self.ctx, C.byref(F), n, pix_sr, C.byref(jy)), "bh_radio_flux")
+        return jy.value
+
+    def vacuum_fates(self, photons, tol, r_escape, max_steps):
+        ph = np.ascontiguousarray(photons, dtype=np.float64).reshape(-1, 9)
+        fates = np.zeros(len(ph), dtype=np.int32)
+        self._check(self.L.bh_vacuum_fates(self.ctx, ph.ctypes.data_as(_D), len(ph), tol, r_escape, max_steps,
+                                           fates.ctypes.data_as(C.POINTER(C.c_int))), "bh_vacuum_fates")
+        return fates
+
+    def find_critical_point(self, F, psi):
+        rho = C.c_double()
+        self._check(self.L.bhh_find_critical_point(self.ctx, C.byref(F), psi, C.byref(rho)),
+                    "bhh_find_critical_point")
+        return rho.value

@@ -1,0 +1,600 @@
+/* bhrt.hip - gfx950 kernels and the C ABI (include/bhrt.h) of the Kerr tracer.
+
+   Launch geometry: a 256-lane workgroup traces a 16x16 pixel tile; each wave64 owns an 8x8
+   sub-tile, so the rays of a wave are image neighbours with similar step counts (less divergence)
+   and similar paths through the turbulence grid (shared cache lines). Workgroups are renumbered so
+   that each of the 8 XCDs (workgroup i runs on XCD i % 8) receives a contiguous band of tiles and
+   the grid lines its rays touch stay in that XCD's L2.
+
+   Per-block partial sums (flux and integrator counters) are reduced by a second kernel in a fixed
+   order, so frame totals are deterministic (no floating-point atomics). */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bhrt.h"
+#include "trace_fp64.h"
+
+using bhd::RayResult;
+using bhd::TraceParams;
+
+namespace {
+
+constexpr int TILE = 16;          // pixels per tile side
+constexpr int BLOCK = TILE * TILE;
+constexpr int NXCD = 8;
+constexpr int NCNT = 5;           // counters per block: attempts, accepted, flow, limited, fp64 attempts
+
+struct BlockOut {
+    double flux;
+    unsigned long long cnt[NCNT];
+};
+
+// tile index of this workgroup after the XCD-aware renumbering; -1 for padding workgroups
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles, int per_xcd) {
+    int t = (bid % NXCD) * per_xcd + bid / NXCD;
+    return t < ntiles ? t : -1;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int PREC, bool RADIO, bool RAYS>
+__global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, float4* __restrict__ px_out,
+                                                 bh_ray_out* __restrict__ rays_out, BlockOut* __restrict__ blk,
+                                                 int per_xcd) {
+    const int tiles_x = (P.w + TILE - 1) / TILE, tiles_y = (P.h + TILE - 1) / TILE;
+    const int tile = xcd_tile(blockIdx.x, tiles_x * tiles_y, per_xcd);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double flux = 0.0;
+    unsigned long long c[NCNT] = {0, 0, 0, 0, 0};
+    if (tile >= 0) {
+        const int lx = (wave & 1) * 8 + (lane & 7), ly = (wave >> 1) * 8 + (lane >> 3);
+        const int tx = (tile % tiles_x) * TILE + lx, ty = (tile / tiles_x) * TILE + ly;
+        if (tx < P.w && ty < P.h) {
+            const int px = P.x0 + tx, py = P.y0 + ty;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            RayResult r;
+            for (int s = 0; s < P.spp; ++s) {
+                double ox, oy;
+                bhd::sample_offsets(P, px, py, s, &ox, &oy);
+                bhd::trace_ray_fp64<RADIO>(P, ox, oy, r);
+                const int fp64_attempts = r.steps;
+                a0 += r.I[0];
+                a1 += r.I[1];
+                a2 += r.I[2];
+                a3 += r.I[3];
+                c[0] += (unsigned long long)r.steps;
+                c[1] += (unsigned long long)r.accepted;
+                c[2] += (unsigned long long)r.flow_steps;
+                c[3] += r.fate == 0 ? 1ull : 0ull;
+                c[4] += (unsigned long long)fp64_attempts;
+            }
+            flux = a0;
+            const size_t o = (size_t)ty * P.w + tx;
+            if (RAYS) {
+                bh_ray_out* q = rays_out + o;
+                for (int i = 0; i < 6; ++i) q->y[i] = r.y[i];
+                q->th_inf = r.th_inf;
+                q->ph_inf = r.ph_inf;
+                q->I[0] = r.I[0]; q->I[1] = r.I[1]; q->I[2] = r.I[2]; q->I[3] = r.I[3];
+                q->trans = r.trans;
+                q->psi = r.psi;
+                q->fate = r.fate;
+                q->ncross = r.ncross;
+                q->order = r.order;
+                q->steps = r.steps;
+                q->accepted = r.accepted;
+                q->flow_steps = r.flow_steps;
+            } else {
+                const double inv = 1.0 / (double)P.spp;
+                px_out[o] = make_float4((float)(a0 * inv), (float)(a1 * inv), (float)(a2 * inv), (float)(a3 * inv));
+            }
+        }
+    }
+    // deterministic block partials: wave64 shuffles, then the 4 waves in order through LDS
+    __shared__ double s_flux[BLOCK / 64];
+    __shared__ unsigned long long s_cnt[BLOCK / 64][NCNT];
+    flux = wave_sum(flux);
+    for (int i = 0; i < NCNT; ++i) c[i] = wave_sum(c[i]);
+    if (lane == 0) {
+        s_flux[wave] = flux;
+        for (int i = 0; i < NCNT; ++i) s_cnt[wave][i] = c[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BlockOut b;
+        b.flux = ((s_flux[0] + s_flux[1]) + s_flux[2]) + s_flux[3];
+        for (int i = 0; i < NCNT; ++i) b.cnt[i] = s_cnt[0][i] + s_cnt[1][i] + s_cnt[2][i] + s_cnt[3][i];
+        blk[blockIdx.x] = b;
+    }
+}
+
+// Sums nb block partials in index order (one workgroup; each lane a strided slice, then a fixed tree).
+__global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk, int nb, BlockOut* __restrict__ tot) {
+    __shared__ double s_f[256];
+    __shared__ unsigned long long s_c[256][NCNT];
+    double f = 0.0;
+    unsigned long long c[NCNT] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        f += blk[i].flux;
+        for (int k = 0; k < NCNT; ++k) c[k] += blk[i].cnt[k];
+    }
+    s_f[threadIdx.x] = f;
+    for (int k = 0; k < NCNT; ++k) s_c[threadIdx.x][k] = c[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_f[threadIdx.x] += s_f[threadIdx.x + w];
+            for (int k = 0; k < NCNT; ++k) s_c[threadIdx.x][k] += s_c[threadIdx.x + w][k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        BlockOut b;
+        b.flux = s_f[0];
+        for (int k = 0; k < NCNT; ++k) b.cnt[k] = s_c[0][k];
+        *tot = b;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_vacuum_fates(const TraceParams P, const double* __restrict__ photons, int n,
+                                                     double tol, double r_escape, int max_steps,
+                                                     int* __restrict__ fates) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ph[9];
+    for (int k = 0; k < 9; ++k) ph[k] = photons[9 * (size_t)i + k];
+    fates[i] = bhd::vacuum_fate(P, ph, tol, r_escape, max_steps);
+}
+
+__global__ void k_math_eval(int fn, const double* __restrict__ x, const double* __restrict__ y,
+                            double* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = x[i], b = y[i], r = 0.0, s, c;
+    switch (fn) {
+        case 0: r = bhm::exp(a); break;
+        case 1: r = bhm::log(a); break;
+        case 2: r = bhm::expm1(a); break;
+        case 3: r = bhm::cbrt(a); break;
+        case 4: r = bhm::pow_m02(a); break;
+        case 5: bhm::sincos(a, &s, &c); r = s; break;
+        case 6: bhm::sincos(a, &s, &c); r = c; break;
+        case 7: r = bhm::atan2(a, b); break;
+        case 8: r = bhm::acos(a); break;
+        case 9: r = sqrt(a); break;
+        default: r = a / b; break;
+    }
+    out[i] = r;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+std::string g_create_error;
+
+}  // namespace
+
+struct bh_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bh_scene S{};
+    bh_settings rs{};
+    float* d_k2 = nullptr;
+    int k2_n = 0;
+    float k2_l0 = 0, k2_dl = 0;
+    float* d_turb = nullptr;
+    int turb_n = 0, turb_nt = 0;
+    double turb_half = 0, turb_t0 = 0, turb_dt = 0;
+    float4* d_px = nullptr;
+    size_t px_cap = 0;
+    bh_ray_out* d_rays = nullptr;
+    size_t rays_cap = 0;
+    BlockOut* d_blk = nullptr;
+    size_t blk_cap = 0;
+    BlockOut* d_tot = nullptr;
+    double* d_ph = nullptr;
+    int* d_fates = nullptr;
+    size_t ph_cap = 0;
+    std::string err;
+};
+
+#define HIPCK(ctx, x)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess) {                                                                            \
+            (ctx)->err = std::string(#x) + ": " + hipGetErrorString(e_);                                   \
+            return BH_ERR_HIP;                                                                             \
+        }                                                                                                  \
+    } while (0)
+
+namespace {
+
+int fail(bh_ctx* c, int code, const std::string& msg) {
+    c->err = msg;
+    return code;
+}
+
+template <typename T>
+int ensure(bh_ctx* c, T** p, size_t* cap, size_t n) {
+    if (*cap >= n && *p) return BH_OK;
+    if (*p) {
+        HIPCK(c, hipStreamSynchronize(c->st));
+        HIPCK(c, hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    HIPCK(c, hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+    *cap = n;
+    return BH_OK;
+}
+
+TraceParams make_params(const bh_ctx* c, const bh_frame* f) {
+    TraceParams P;
+    std::memset(&P, 0, sizeof P);
+    const bh_scene& S = c->S;
+    P.a = S.a;
+    P.a2 = S.a * S.a;
+    P.rh = S.rh;
+    P.r_isco = S.r_isco;
+    P.tol = S.tol;
+    P.tol4 = S.tol * 4.0;
+    P.r_escape = S.r_escape;
+    P.r_stop = S.r_stop;
+    P.step_frac = S.step_frac;
+    P.max_steps = S.max_steps;
+    P.mode = S.mode;
+    P.flow_on = S.flow_on;
+    P.turb_on = (S.flow_turb_amp > 0.f && c->d_turb && c->turb_n > 1) ? 1 : 0;
+    P.n0 = S.n0;
+    P.n_index = S.n_index;
+    P.theta_index = S.theta_index;
+    P.theta0 = S.theta0;
+    P.h_r = S.h_r;
+    P.beta_p = S.beta_p;
+    P.r_flow_max = S.r_flow_max;
+    P.nu_obs = S.nu_obs;
+    P.rg_cm = S.rg_cm;
+    P.flow_kep = S.flow_kep;
+    P.k2 = c->d_k2;
+    P.k2_n = c->k2_n;
+    P.k2_l0 = c->k2_l0;
+    P.k2_dl = c->k2_dl;
+    P.turb = c->d_turb;
+    P.turb_n = c->turb_n;
+    P.turb_nt = c->turb_nt;
+    P.turb_half = c->turb_half;
+    P.turb_inv_dx = c->turb_n > 0 ? c->turb_n / (2.0 * c->turb_half) : 0.0;
+    P.turb_t0 = c->turb_t0;
+    P.turb_inv_dt = c->turb_dt > 0 ? 1.0 / c->turb_dt : 0.0;
+    if (f) {
+        P.ph0 = f->ph;
+        P.t0 = f->t;
+        for (int i = 0; i < 11; ++i) P.kp[i] = f->kp[i];
+        for (int i = 0; i < 3; ++i) {
+            P.F[i] = f->F[i];
+            P.R[i] = f->R[i];
+            P.U[i] = f->U[i];
+            P.beta[i] = f->beta[i];
+        }
+        P.cx = f->cx;
+        P.cy = f->cy;
+        P.pix = f->pix;
+        P.W = f->W;
+        P.H = f->H;
+        P.seed = f->seed;
+    }
+    return P;
+}
+
+int check_frame(bh_ctx* c, const bh_frame* f) {
+    if (!f) return fail(c, BH_ERR_ARG, "frame is NULL");
+    if (f->W <= 0 || f->H <= 0 || f->W > 65536 || f->H > 65536) return fail(c, BH_ERR_ARG, "bad frame size");
+    if (!(f->pix > 0.0)) return fail(c, BH_ERR_ARG, "frame pixel scale must be > 0");
+    if (c->S.mode == BH_MODE_RADIO && c->S.flow_on && !c->d_k2) return fail(c, BH_ERR_ARG, "radio mode needs the K2 table");
+    return BH_OK;
+}
+
+using KernelFn = void (*)(const TraceParams, float4*, bh_ray_out*, BlockOut*, int);
+
+KernelFn pick_kernel(int prec, bool radio, bool rays) {
+    if (prec == BH_PREC_FP64) {
+        if (radio) return rays ? k_trace<BH_PREC_FP64, true, true> : k_trace<BH_PREC_FP64, true, false>;
+        return rays ? k_trace<BH_PREC_FP64, false, true> : k_trace<BH_PREC_FP64, false, false>;
+    }
+    return nullptr;
+}
+
+// Launches the trace of a w x h tile at (x0, y0) and the block reduction; fills stats counters.
+int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0, int y0, int w, int h, bool rays,
+                 bh_stats* st) {
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0) return fail(c, BH_ERR_ARG, "empty tile");
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    if (o.precision != BH_PREC_FP64 && o.precision != BH_PREC_MIXED) return fail(c, BH_ERR_ARG, "bad precision");
+    if (o.precision == BH_PREC_MIXED) return fail(c, BH_ERR_UNSUPPORTED, "mixed precision is not built yet");
+    if (o.emissivity != BH_EMIS_EXACT) return fail(c, BH_ERR_UNSUPPORTED, "MLP emissivity is not built yet");
+    TraceParams P = make_params(c, f);
+    P.x0 = x0;
+    P.y0 = y0;
+    P.w = w;
+    P.h = h;
+    P.spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
+    if (rays) P.spp = 1;
+    P.jitter = o.jitter;
+    const int ntiles = ((w + TILE - 1) / TILE) * ((h + TILE - 1) / TILE);
+    const int per_xcd = (ntiles + NXCD - 1) / NXCD;
+    const int nb = per_xcd * NXCD;
+    rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)nb);
+    if (rc) return rc;
+    if (rays) rc = ensure(c, &c->d_rays, &c->rays_cap, (size_t)w * h);
+    else rc = ensure(c, &c->d_px, &c->px_cap, (size_t)w * h);
+    if (rc) return rc;
+    KernelFn k = pick_kernel(o.precision, c->S.mode == BH_MODE_RADIO, rays);
+    HIPCK(c, hipEventRecord(c->ev0, c->st));
+    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, c->d_px, c->d_rays, c->d_blk, per_xcd);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipEventRecord(c->ev1, c->st));
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, nb, c->d_tot);
+    HIPCK(c, hipGetLastError());
+    BlockOut tot;
+    HIPCK(c, hipMemcpyAsync(&tot, c->d_tot, sizeof tot, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (st) {
+        float ms = 0.f;
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->ms_kernel = ms;
+        st->samples = (long long)w * h * P.spp;
+        st->step_attempts = (long long)tot.cnt[0];
+        st->steps_accepted = (long long)tot.cnt[1];
+        st->flow_steps = (long long)tot.cnt[2];
+        st->step_limited = (long long)tot.cnt[3];
+        st->fp64_steps = (long long)tot.cnt[4];
+        st->flux_sum = tot.flux;
+    }
+    return BH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const bh_settings* settings, bh_ctx** out,
+              char* err, size_t errlen) {
+    auto set_err = [&](const std::string& m) {
+        g_create_error = m;
+        if (err && errlen) {
+            std::strncpy(err, m.c_str(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+    };
+    if (!out || !scene) {
+        set_err("bh_create: NULL argument");
+        return BH_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = bh_device_count();
+    if (ndev <= 0) {
+        set_err("bh_create: no HIP device");
+        return BH_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        set_err("bh_create: device index out of range");
+        return BH_ERR_ARG;
+    }
+    bh_ctx* c = new bh_ctx;
+    c->device = device;
+    c->S = *scene;
+    if (settings) c->rs = *settings;
+    auto bail = [&](int code) {
+        set_err(c->err);
+        bh_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) {
+        c->err = "hipSetDevice failed";
+        return bail(BH_ERR_HIP);
+    }
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc((void**)&c->d_tot, sizeof(BlockOut)) != hipSuccess) {
+        c->err = "stream/event/alloc creation failed";
+        return bail(BH_ERR_HIP);
+    }
+    if (tables) {
+        if (tables->k2_ln && tables->k2_n >= 2) {
+            c->k2_n = tables->k2_n;
+            c->k2_l0 = tables->k2_l0;
+            c->k2_dl = tables->k2_dl;
+            if (hipMalloc((void**)&c->d_k2, sizeof(float) * c->k2_n) != hipSuccess ||
+                hipMemcpy(c->d_k2, tables->k2_ln, sizeof(float) * c->k2_n, hipMemcpyHostToDevice) != hipSuccess) {
+                c->err = "K2 table upload failed";
+                return bail(BH_ERR_HIP);
+            }
+        }
+        if (tables->turb && tables->turb_n >= 2) {
+            int nt = tables->turb_nt > 0 ? tables->turb_nt : 1;
+            if (nt > 1 && !(tables->turb_dt > 0.0)) {
+                c->err = "turbulence time slices need turb_dt > 0";
+                return bail(BH_ERR_ARG);
+            }
+            if (!(tables->turb_half > 0.0)) {
+                c->err = "turbulence grid needs turb_half > 0";
+                return bail(BH_ERR_ARG);
+            }
+            size_t n = (size_t)tables->turb_n * tables->turb_n * tables->turb_n * nt;
+            c->turb_n = tables->turb_n;
+            c->turb_nt = nt;
+            c->turb_half = tables->turb_half;
+            c->turb_t0 = tables->turb_t0;
+            c->turb_dt = tables->turb_dt;
+            if (hipMalloc((void**)&c->d_turb, sizeof(float) * n) != hipSuccess ||
+                hipMemcpy(c->d_turb, tables->turb, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess) {
+                c->err = "turbulence grid upload failed";
+                return bail(BH_ERR_HIP);
+            }
+        }
+    }
+    *out = c;
+    return BH_OK;
+}
+
+int bh_update_scene(bh_ctx* c, const bh_scene* s) {
+    if (!c || !s) return BH_ERR_ARG;
+    c->S = *s;
+    return BH_OK;
+}
+
+int bh_get_scene(const bh_ctx* c, bh_scene* s) {
+    if (!c || !s) return BH_ERR_ARG;
+    *s = c->S;
+    return BH_OK;
+}
+
+int bh_render(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, float* px_out, bh_stats* st) {
+    if (!c) return BH_ERR_ARG;
+    double t0 = now_ms();
+    HIPCK(c, hipSetDevice(c->device));
+    bh_stats local{};
+    int rc = launch_trace(c, f, opts, 0, 0, f ? f->W : 0, f ? f->H : 0, false, &local);
+    if (rc) return rc;
+    if (px_out) {
+        HIPCK(c, hipMemcpyAsync(px_out, c->d_px, sizeof(float4) * (size_t)f->W * f->H, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+    }
+    local.ms_trace = now_ms() - t0;
+    if (st) *st = local;
+    return BH_OK;
+}
+
+int bh_render_resident(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, const void** d_px, bh_stats* st) {
+    if (!c) return BH_ERR_ARG;
+    double t0 = now_ms();
+    HIPCK(c, hipSetDevice(c->device));
+    bh_stats local{};
+    int rc = launch_trace(c, f, opts, 0, 0, f ? f->W : 0, f ? f->H : 0, false, &local);
+    if (rc) return rc;
+    if (d_px) *d_px = c->d_px;
+    local.ms_trace = now_ms() - t0;
+    if (st) *st = local;
+    return BH_OK;
+}
+
+int bh_trace_tile(bh_ctx* c, const bh_frame* f, int x0, int y0, int w, int h, const bh_trace_opts* opts,
+                  bh_ray_out* out) {
+    if (!c) return BH_ERR_ARG;
+    if (!out) return fail(c, BH_ERR_ARG, "output is NULL");
+    HIPCK(c, hipSetDevice(c->device));
+    int rc = launch_trace(c, f, opts, x0, y0, w, h, true, nullptr);
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(out, c->d_rays, sizeof(bh_ray_out) * (size_t)w * h, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return BH_OK;
+}
+
+int bh_radio_flux(bh_ctx* c, const bh_frame* f, int n, double pix_sr, double* jy) {
+    if (!c || !f || !jy || n <= 0) return BH_ERR_ARG;
+    HIPCK(c, hipSetDevice(c->device));
+    bh_frame g = *f;
+    g.W = n;
+    g.H = n;
+    g.spp = 2;
+    bh_trace_opts o{};
+    o.precision = BH_PREC_FP64;
+    o.jitter = 1;
+    o.spp = 2;
+    bh_stats st{};
+    int rc = launch_trace(c, &g, &o, 0, 0, n, n, false, &st);
+    if (rc) return rc;
+    // render.cu:803-808: sum of the per-pixel means times the pixel solid angle, in Jy
+    *jy = st.flux_sum / 2.0 * pix_sr / 1e-23;
+    return BH_OK;
+}
+
+int bh_vacuum_fates(bh_ctx* c, const double* photons, int n, double tol, double r_escape, int max_steps, int* fates) {
+    if (!c || !photons || !fates || n < 0) return BH_ERR_ARG;
+    if (n == 0) return BH_OK;
+    HIPCK(c, hipSetDevice(c->device));
+    if (c->ph_cap < (size_t)n) {
+        if (c->d_ph) {
+            HIPCK(c, hipFree(c->d_ph));
+            HIPCK(c, hipFree(c->d_fates));
+        }
+        c->d_ph = nullptr;
+        c->d_fates = nullptr;
+        c->ph_cap = 0;
+        HIPCK(c, hipMalloc((void**)&c->d_ph, sizeof(double) * 9 * n));
+        HIPCK(c, hipMalloc((void**)&c->d_fates, sizeof(int) * n));
+        c->ph_cap = n;
+    }
+    HIPCK(c, hipMemcpyAsync(c->d_ph, photons, sizeof(double) * 9 * n, hipMemcpyHostToDevice, c->st));
+    TraceParams P = make_params(c, nullptr);
+    hipLaunchKernelGGL(k_vacuum_fates, dim3((n + 63) / 64), dim3(64), 0, c->st, P, c->d_ph, n, tol, r_escape,
+                       max_steps, c->d_fates);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipMemcpyAsync(fates, c->d_fates, sizeof(int) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return BH_OK;
+}
+
+int bh_math_eval(int fn, const double* x, const double* y, double* out, int n) {
+    if (!x || !y || !out || n < 0 || fn < 0 || fn > 10) return BH_ERR_ARG;
+    if (n == 0) return BH_OK;
+    if (bh_device_count() <= 0) return BH_ERR_NODEVICE;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    size_t bytes = sizeof(double) * (size_t)n;
+    int rc = BH_OK;
+    if (hipSetDevice(0) != hipSuccess || hipMalloc((void**)&dx, bytes) != hipSuccess ||
+        hipMalloc((void**)&dy, bytes) != hipSuccess || hipMalloc((void**)&dout, bytes) != hipSuccess ||
+        hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = BH_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(k_math_eval, dim3((n + 255) / 256), dim3(256), 0, 0, fn, dx, dy, dout, n);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = BH_ERR_HIP;
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}
+
+void bh_destroy(bh_ctx* c) {
+    if (!c) return;
+    // best-effort teardown: errors here have nowhere to go
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    void* bufs[] = {c->d_k2, c->d_turb, c->d_px, c->d_rays, c->d_blk, c->d_tot, c->d_ph, c->d_fates};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* bh_last_error(const bh_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

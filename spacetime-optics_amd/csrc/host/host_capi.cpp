@@ -1,0 +1,235 @@
+/* host_capi.cpp - C entry points of the host layer (include/bhrt_host.h). */
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../../include/bhrt_host.h"
+#include "config.h"
+#include "renderer.h"
+#include "scene.h"
+#include "shot.h"
+
+namespace {
+
+void copy_err(const std::string& m, char* err, size_t len) {
+    if (err && len) {
+        std::strncpy(err, m.c_str(), len - 1);
+        err[len - 1] = 0;
+    }
+}
+
+bool load(const char* ini, const char* sets, bh::Config* c, std::string* err) {
+    if (!ini || !c->load(ini, err)) return false;
+    std::string s = sets ? sets : "";
+    size_t p = 0;
+    while (p < s.size()) {
+        size_t q = s.find(';', p);
+        if (q == std::string::npos) q = s.size();
+        c->apply_set(s.substr(p, q - p));
+        p = q + 1;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhh_config_str(const char* ini, const char* sets, const char* key, char* buf, int len) {
+    bh::Config c;
+    std::string err;
+    if (!load(ini, sets, &c, &err)) {
+        copy_err(err, buf, len);
+        return -1;
+    }
+    if (!c.has(key)) return 0;
+    copy_err(c.str(key), buf, len);
+    return 1;
+}
+
+int bhh_build_scene(const char* ini, const char* sets, bh_scene* scene, float* k2, float* k2_l0, float* k2_dl,
+                    double* info, bh_trace_opts* opts, char* err, size_t errlen) {
+    bh::Config c;
+    std::string e;
+    if (!load(ini, sets, &c, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_ARG;
+    }
+    bh::SceneTables T;
+    bh::SceneInfo I;
+    bh::TracerOptions O;
+    bh_scene S;
+    if (!bh::build_scene(c, &S, &T, &I, &O, &e, false)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_ARG;
+    }
+    if (scene) *scene = S;
+    if (k2) std::memcpy(k2, T.k2_ln.data(), sizeof(float) * T.k2_ln.size());
+    if (k2_l0) *k2_l0 = T.k2_l0;
+    if (k2_dl) *k2_dl = T.k2_dl;
+    if (info) {
+        double v[6] = {I.mass_msun, I.distance_cm, I.rg_cm, I.uas_per_m, I.radio_flux_target_jy, I.normalize_fov_m};
+        std::memcpy(info, v, sizeof v);
+    }
+    if (opts) {
+        std::memset(opts, 0, sizeof(*opts));
+        opts->precision = O.precision;
+        opts->emissivity = O.emissivity;
+        opts->jitter = O.jitter;
+        opts->spp = c.integer("render.spp", 8);
+    }
+    return BH_OK;
+}
+
+void bhh_turbulence(int n, int nt, double half, double sigma, double corr_m, double corr_t, double dt, uint32_t seed,
+                    float* out) {
+    std::vector<float> v;
+    bh::make_turbulence(n, nt, half, sigma, corr_m, corr_t, dt, seed, &v);
+    std::memcpy(out, v.data(), sizeof(float) * v.size());
+}
+
+int bhh_build_shot(const char* ini, const char* sets, double* out, int maxn) {
+    bh::Config c;
+    std::string e;
+    if (!load(ini, sets, &c, &e)) return -1;
+    bh::Shot sh;
+    if (!bh::build_shot(c, &sh, &e)) return -1;
+    int n = (int)sh.frames.size();
+    for (int i = 0; i < n && i < maxn; ++i) {
+        const bh::ShotFrame& f = sh.frames[i];
+        double v[14] = {f.t, f.r, f.incl, f.phi, f.roll, f.yaw, f.pitch, f.fov, f.fov_in_m ? 1.0 : 0.0,
+                        f.zoom, f.target, f.off_x, f.off_y, f.time_m};
+        std::memcpy(out + 14 * (size_t)i, v, sizeof v);
+    }
+    return n;
+}
+
+void bhh_make_frame(double a, double r, double incl, double phi, double roll, double yaw, double pitch, double fov,
+                    int fov_in_m, double zoom, double target, double target_x, double target_y, double off_x,
+                    double off_y, double time_m, int orbit, int W, int H, int spp, uint32_t seed, bh_frame* out) {
+    bh::Shot sh;
+    sh.motion = orbit ? "orbit" : "static";
+    sh.target_x = target_x;
+    sh.target_y = target_y;
+    bh::ShotFrame f;
+    f.r = r; f.incl = incl; f.phi = phi; f.roll = roll; f.yaw = yaw; f.pitch = pitch; f.fov = fov;
+    f.fov_in_m = fov_in_m != 0; f.zoom = zoom; f.target = target; f.off_x = off_x; f.off_y = off_y;
+    f.time_m = time_m;
+    *out = bh::make_frame(sh, f, a, W, H, spp, seed);
+}
+
+int bhh_find_critical_point(bh_ctx* ctx, const bh_frame* frame, double psi, double* rho) {
+    if (!ctx || !frame || !rho) return BH_ERR_ARG;
+    bh::FateFn fates = [&](const std::vector<double>& ph, std::vector<int>* out) {
+        int n = (int)(ph.size() / 9);
+        out->assign(n, 0);
+        return bh_vacuum_fates(ctx, ph.data(), n, 1e-12, 200.0, 2000000, out->data()) == BH_OK;
+    };
+    std::vector<double> r;
+    bh_scene S;
+    bh_get_scene(ctx, &S);
+    if (!bh::find_critical_points(S.a, {*frame}, {psi}, fates, &r)) return BH_ERR_HIP;
+    *rho = r[0];
+    return BH_OK;
+}
+
+int bhh_create_from_ini(const char* ini, const char* sets, int device, int normalize, int frame_index, bh_ctx** ctx,
+                        bh_scene* scene, bh_frame* frame, bh_trace_opts* opts, double* jy, char* err, size_t errlen) {
+    if (!ctx) return BH_ERR_ARG;
+    *ctx = nullptr;
+    bh::Config c;
+    std::string e;
+    if (!load(ini, sets, &c, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_ARG;
+    }
+    bh::SceneTables T;
+    bh::SceneInfo I;
+    bh::TracerOptions O;
+    bh_scene S;
+    bh::Shot shot;
+    if (!bh::build_scene(c, &S, &T, &I, &O, &e) || !bh::build_shot(c, &shot, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_ARG;
+    }
+    if (frame_index < 0 || frame_index >= (int)shot.frames.size()) {
+        copy_err("frame index out of range", err, errlen);
+        return BH_ERR_ARG;
+    }
+    bh_settings rs{};
+    rs.W = c.integer("camera.width", 1920);
+    rs.H = c.integer("camera.height", 1080);
+    rs.spp = c.integer("render.spp", 8);
+    rs.spp_max = c.integer("render.spp_max", 128);
+    rs.target_err = (float)c.num("render.target_error", 0.03);
+    bh::Renderer R;
+    if (!R.init(device, S, T, rs, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_NODEVICE;
+    }
+    // zoom target on the critical curve (main.cpp:421-428)
+    if (shot.type == "render" && c.has("camera.target_psi_deg")) {
+        double psi = c.num("camera.target_psi_deg", 0) * bh::PI / 180;
+        bh_frame F0 = bh::make_frame(shot, shot.frames[0], S.a, rs.W, rs.H, rs.spp, 0);
+        bh_scene Sv = S;   // the search runs with its own tolerance; the scene's spin is what matters
+        bh::FateFn fates = [&](const std::vector<double>& ph, std::vector<int>* out) {
+            return R.vacuum_fates(ph, 1e-12, 200.0, 2000000, out);
+        };
+        std::vector<double> rho;
+        if (!bh::find_critical_points(Sv.a, {F0}, {psi}, fates, &rho)) {
+            copy_err("critical point search failed: " + R.error(), err, errlen);
+            return BH_ERR_HIP;
+        }
+        double r = rho[0] + c.num("camera.target_offset_m", 0.0);
+        shot.target_x = r * std::cos(psi);
+        shot.target_y = r * std::sin(psi);
+    }
+    double got = 0;
+    if (normalize && S.mode == BH_MODE_RADIO && I.radio_flux_target_jy > 0) {
+        if (!bh::normalize_flux(R, shot, shot.frames[0], I, &S, &got, &e)) {
+            copy_err(e, err, errlen);
+            return BH_ERR_HIP;
+        }
+    }
+    if (jy) *jy = got;
+    if (scene) *scene = S;
+    if (frame)
+        *frame = bh::make_frame(shot, shot.frames[frame_index], S.a, rs.W, rs.H, rs.spp,
+                                0x9E3779B9u * (uint32_t)(frame_index + 1));
+    if (opts) {
+        std::memset(opts, 0, sizeof(*opts));
+        opts->precision = O.precision;
+        opts->emissivity = O.emissivity;
+        opts->jitter = O.jitter;
+        opts->spp = rs.spp;
+    }
+    *ctx = R.release();
+    return BH_OK;
+}
+
+int bhh_selftest(int device, double* rows, char* err, size_t errlen) {
+    bh::Renderer R;
+    bh_scene S{};
+    S.a = 0.5;
+    S.rh = bh::horizon_radius(S.a);
+    S.r_isco = bh::isco_radius(S.a);
+    S.tol = 1e-12;
+    S.max_steps = 2000000;
+    bh::SceneTables T;
+    bh_settings rs{};
+    std::string e;
+    if (!R.init(device, S, T, rs, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_NODEVICE;
+    }
+    std::vector<std::vector<double>> out;
+    if (!bh::selftest_rows(R, &out, &e)) {
+        copy_err(e, err, errlen);
+        return BH_ERR_HIP;
+    }
+    for (size_t i = 0; i < out.size(); ++i)
+        for (int k = 0; k < 6; ++k) rows[6 * i + k] = out[i][k];
+    return BH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+/* bhrt - command line front end on the MI355X tracer (blackholes/src/main.cpp:374-626 of the
+   reference, single GPU): bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B]
+   [--list] [--selftest]. Frames are written as linear HDR (I, I_n1, I_n2, I_n>=3) float images
+   "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first; tone mapping, HUD and PNG output are not
+   part of this path. */
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/bhrt_host.h"
+#include "config.h"
+#include "renderer.h"
+#include "scene.h"
+#include "shot.h"
+
+namespace {
+
+double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B] [--list] [--selftest]\n");
+}
+
+void mkdirs(const std::string& p) {
+    std::string cur;
+    for (size_t i = 0; i < p.size(); ++i) {
+        cur += p[i];
+        if (p[i] == '/' || i + 1 == p.size()) mkdir(cur.c_str(), 0755);
+    }
+}
+
+bool write_frame(const std::string& path, const bh::HdrFrame& f) {
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    int32_t hdr[3] = {f.W, f.H, 4};
+    bool ok = std::fwrite("BHF1", 1, 4, fp) == 4 && std::fwrite(hdr, 4, 3, fp) == 3 &&
+              std::fwrite(f.px.data(), 4, f.px.size(), fp) == f.px.size();
+    return std::fclose(fp) == 0 && ok;
+}
+
+int selftest() {
+    std::printf("self-test: shadow edge from ray tracing vs analytic critical curve\n");
+    std::vector<double> rows(48);
+    char err[512] = {0};
+    if (bhh_selftest(0, rows.data(), err, sizeof err) != BH_OK) {
+        std::fprintf(stderr, "bhrt: self-test failed: %s\n", err);
+        return 1;
+    }
+    for (int i = 0; i < 8; ++i) {
+        const double* r = &rows[6 * i];
+        std::printf("  a=%.2f incl=%4.0f  ray-traced edges %+.6f %+.6f   analytic %+.6f %+.6f  (M)\n", r[0], r[1], r[2],
+                    r[3], r[4], r[5]);
+    }
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    double t_start = wall();
+    if (argc < 2) {
+        usage();
+        return 1;
+    }
+    if (std::string(argv[1]) == "--selftest") return selftest();
+    bh::Config cfg;
+    std::string err;
+    if (!cfg.load(argv[1], &err)) {
+        std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+        return 1;
+    }
+    bool list = false;
+    int fa = 0, fb = -1;
+    for (int i = 2; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--set" && i + 1 < argc) cfg.apply_set(argv[++i]);
+        else if (a == "--out" && i + 1 < argc) cfg.set("output.dir", argv[++i]);
+        else if (a == "--frames" && i + 1 < argc) std::sscanf(argv[++i], "%d:%d", &fa, &fb);
+        else if (a == "--list") list = true;
+        else {
+            usage();
+            return 1;
+        }
+    }
+    bh_scene S;
+    bh::SceneTables T;
+    bh::SceneInfo si;
+    bh::TracerOptions opt;
+    bh::Shot shot;
+    if (!bh::build_scene(cfg, &S, &T, &si, &opt, &err, !list) || !bh::build_shot(cfg, &shot, &err)) {
+        std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+        return 1;
+    }
+    bh_settings rs{};
+    rs.W = cfg.integer("camera.width", 1920);
+    rs.H = cfg.integer("camera.height", 1080);
+    rs.spp = cfg.integer("render.spp", 8);
+    rs.spp_max = cfg.integer("render.spp_max", 128);
+    rs.target_err = (float)cfg.num("render.target_error", 0.03);
+    int nframes = (int)shot.frames.size();
+    if (fb < 0 || fb > nframes) fb = nframes;
+    fa = std::max(0, std::min(fa, fb));
+    std::printf("bhrt | 1 GPU | shot '%s' (%s) %d frames %dx%d spp %d | %s%s\n", shot.name.c_str(), shot.type.c_str(),
+                nframes, rs.W, rs.H, rs.spp, opt.precision == BH_PREC_MIXED ? "mixed" : "fp64",
+                opt.emissivity == BH_EMIS_MLP ? " + mlp" : "");
+    std::printf("  a = %.4f, r_h = %.4f M, r_isco = %.4f M, %s, M = %.3g Msun (GM/c^2 = %.4g km, %.3g uas)\n", S.a, S.rh,
+                S.r_isco, S.mode == BH_MODE_RADIO ? "radio" : "visible", si.mass_msun, si.rg_cm * 1e-5, si.uas_per_m);
+    if (list) {
+        for (const bh::ShotFrame& f : shot.frames)
+            std::printf("  frame %4d t=%6.2f r=%.4g incl=%.1f phi=%.1f fov=%.4g zoom=%.4g target=%.2f time=%.1f\n",
+                        f.index, f.t, f.r, f.incl * 180 / bh::PI, f.phi * 180 / bh::PI,
+                        f.fov_in_m ? f.fov : f.fov * 180 / bh::PI, f.zoom, f.target, f.time_m);
+        return 0;
+    }
+    bh::Renderer R;
+    if (bh_device_count() <= 0) {
+        std::fprintf(stderr, "bhrt: no HIP device\n");
+        return 1;
+    }
+    double t0 = wall();
+    if (!R.init(0, S, T, rs, &err)) {
+        std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+        return 1;
+    }
+    std::printf("  GPU init %.2f s\n", wall() - t0);
+    if (shot.type == "render" && cfg.has("camera.target_psi_deg")) {
+        double psi = cfg.num("camera.target_psi_deg", 0) * bh::PI / 180;
+        bh_frame F0 = bh::make_frame(shot, shot.frames[0], S.a, rs.W, rs.H, rs.spp, 0);
+        bh::FateFn fates = [&](const std::vector<double>& ph, std::vector<int>* out) {
+            return R.vacuum_fates(ph, 1e-12, 200.0, 2000000, out);
+        };
+        std::vector<double> rho;
+        if (!bh::find_critical_points(S.a, {F0}, {psi}, fates, &rho)) {
+            std::fprintf(stderr, "bhrt: %s\n", R.error().c_str());
+            return 1;
+        }
+        double r = rho[0] + cfg.num("camera.target_offset_m", 0.0);
+        shot.target_x = r * std::cos(psi);
+        shot.target_y = r * std::sin(psi);
+        std::printf("  zoom target: critical curve at psi = %.1f deg, rho = %.12f M\n", psi * 180 / bh::PI, r);
+    }
+    if (S.mode == BH_MODE_RADIO && si.radio_flux_target_jy > 0) {
+        double got = 0;
+        if (!bh::normalize_flux(R, shot, shot.frames[0], si, &S, &got, &err)) {
+            std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+            return 1;
+        }
+        std::printf("  flow normalised: n0 = %.4g cm^-3 for %.3g Jy (got %.3g Jy)\n", S.n0, si.radio_flux_target_jy, got);
+    }
+    std::string out = cfg.str("output.dir", "out/" + shot.name);
+    mkdirs(out + "/frames");
+    bh_trace_opts o{};
+    o.precision = opt.precision;
+    o.emissivity = opt.emissivity;
+    o.jitter = opt.jitter;
+    o.spp = rs.spp;
+    long long samples = 0, limited = 0;
+    double t_render = wall();
+    for (int idx = fa; idx < fb; ++idx) {
+        bh_frame F = bh::make_frame(shot, shot.frames[idx], S.a, rs.W, rs.H, rs.spp, 0x9E3779B9u * (uint32_t)(idx + 1));
+        bh::HdrFrame hdr;
+        bh_stats st{};
+        double t1 = wall();
+        if (!R.render(F, o, &hdr, &st)) {
+            std::fprintf(stderr, "bhrt: frame %d: %s\n", idx, R.error().c_str());
+            return 1;
+        }
+        samples += st.samples;
+        limited += st.step_limited;
+        char name[64];
+        std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", idx);
+        if (!write_frame(out + name, hdr)) {
+            std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+            return 1;
+        }
+        std::printf("[gpu 0] %s frame %4d  %6.1f Msamples  %7.1f ms  (step-limited %lld)\n", shot.name.c_str(), idx,
+                    st.samples * 1e-6, (wall() - t1) * 1e3, st.step_limited);
+        std::fflush(stdout);
+    }
+    double tr = wall() - t_render;
+    FILE* m = std::fopen((out + "/meta.json").c_str(), "w");
+    if (m) {
+        std::fprintf(m, "{\"shot\": \"%s\", \"frames\": %d, \"fps\": %g, \"width\": %d, \"height\": %d}\n", shot.name.c_str(),
+                     nframes, shot.fps, rs.W, rs.H);
+        std::fclose(m);
+    }
+    std::printf("done: %d frames, %.3g samples (%lld step-limited) in %.1f s render (%.3g Msamples/s), %.1f s total\n",
+                fb - fa, (double)samples, limited, tr, samples / std::max(tr, 1e-9) * 1e-6, wall() - t_start);
+    return 0;
+}

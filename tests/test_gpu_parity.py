@@ -1,0 +1,199 @@
+"""GPU parity of the plain FP64 tracer against the CPU oracle (bh_math build), through the C ABI.
+
+Bar: bit-exact for every per-ray output (final state, escape angles, fate, winding count, order,
+step counts, FP64 intensities) on the same inputs; frame flux to 1e-12 relative (summation order).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import bhrt_amd as bh
+from helpers import D2R, pixel_grid, setup, shot, tracer_for
+
+pytestmark = pytest.mark.gpu
+
+RAY_FIELDS = ["y", "th_inf", "ph_inf", "I", "trans", "psi", "fate", "ncross", "order", "steps", "accepted",
+              "flow_steps"]
+
+
+def assert_rays_equal(gpu, ref, what):
+    gpu = gpu.ravel()
+    ref = ref.ravel()
+    for f in RAY_FIELDS:
+        a, b = gpu[f], ref[f]
+        if a.dtype.kind == "f":
+            neq = (a.view(np.int64) != b.view(np.int64)).reshape(len(a), -1).any(axis=1)
+        else:
+            neq = (a != b).reshape(len(a), -1).any(axis=1)
+        bad = np.nonzero(neq)[0]
+        assert len(bad) == 0, f"{what}: field {f} differs at {len(bad)} rays, first {bad[:5]}: {a[bad[:3]]} vs {b[bad[:3]]}"
+
+
+def oracle_rays(oracle, cfg, px, py):
+    _, rays, _ = oracle.trace_pixels(cfg["OS"], cfg["OF"], px, py, spp=1, jitter=False, nthreads=0)
+    return rays
+
+
+@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos", "sqrt",
+                                "div"])
+def test_device_math_bit_exact(fn, oracle_bhm):
+    rng = np.random.default_rng(7)
+    n = 200000
+    if fn in ("exp", "expm1"):
+        x = np.concatenate([rng.uniform(-745, 710, n // 2), rng.uniform(-1, 1, n // 4) * 10.0 ** rng.uniform(-18, 0, n // 4),
+                            rng.uniform(-40, 40, n // 4)])
+    elif fn in ("log", "cbrt", "pow_m02", "sqrt"):
+        x = np.concatenate([10.0 ** rng.uniform(-300, 300, n // 2), rng.uniform(0, 4, n // 2)])
+    elif fn in ("sin", "cos"):
+        x = np.concatenate([rng.uniform(-400, 400, n // 2), rng.uniform(-1e5, 1e5, n // 2)])
+    elif fn == "acos":
+        x = rng.uniform(-1, 1, n)
+    else:
+        x = rng.normal(size=n) * 10.0 ** rng.uniform(-5, 5, n)
+    y = rng.normal(size=n) * 10.0 ** rng.uniform(-5, 5, n)
+    got = bh.math_eval(fn, x, y)
+    want = oracle_bhm.math_eval(fn, x, y)
+    assert np.array_equal(got.view(np.int64), want.view(np.int64)), f"{fn}: {np.sum(got != want)} mismatches"
+    if fn == "sqrt":
+        assert np.array_equal(got, np.sqrt(x))   # correctly rounded
+    if fn == "div":
+        assert np.array_equal(got, x / y)
+
+
+def test_vacuum_frame_c1_bit_exact(oracle_bhm):
+    """Config 1: the whole 256 x 256 vacuum frame, every per-ray output bit for bit."""
+    cfg = setup(shot("c1_vacuum_256.ini"))
+    t = tracer_for(cfg)
+    gpu = t.trace_tile(cfg["F"], 0, 0, 256, 256)
+    px, py = pixel_grid(256, 256)
+    ref = oracle_rays(oracle_bhm, cfg, px, py)
+    assert_rays_equal(gpu, ref, "c1 256^2")
+    fates = gpu["fate"].ravel()
+    assert set(np.unique(fates)) == {-1, 1}          # no step-limited rays
+    assert 400 < gpu["steps"].mean() < 900           # ~600 DP54 steps per ray
+
+
+def test_vacuum_c1_against_glibc_oracle(oracle_libm):
+    """Against the glibc oracle (bit-identical to the reference's host integrator): fates and
+    winding counts exact, final positions within 1e-9 relative."""
+    cfg = setup(shot("c1_vacuum_256.ini"))
+    t = tracer_for(cfg)
+    gpu = t.trace_tile(cfg["F"], 0, 0, 256, 256).ravel()
+    px, py = pixel_grid(256, 256)
+    ref = oracle_rays(oracle_libm, cfg, px, py)
+    assert np.array_equal(gpu["fate"], ref["fate"])
+    assert np.array_equal(gpu["ncross"], ref["ncross"])
+    for k in (0, 2, 4):   # r, cos(theta), phi
+        a, b = gpu["y"][:, k], ref["y"][:, k]
+        assert np.all(np.abs(a - b) <= 1e-9 * np.maximum(1.0, np.abs(b))), k
+    esc = gpu["fate"] == 1
+    assert np.all(np.abs(gpu["th_inf"][esc] - ref["th_inf"][esc]) <= 1e-9 * np.maximum(1, np.abs(ref["th_inf"][esc])))
+
+
+def test_radio_tiles_c2_bit_exact(oracle_bhm):
+    """Config 2 (1024^2 radio frame): a 64 x 64 block across the photon ring plus 1024 scattered
+    pixels; intensities, transmission, swept angle and counters bit for bit."""
+    cfg = setup(shot("c2_radio_1024.ini"))
+    t = tracer_for(cfg)
+    x0, y0 = 512 - 32 + 96, 512 - 32 - 40
+    gpu = t.trace_tile(cfg["F"], x0, y0, 64, 64)
+    px, py = pixel_grid(64, 64, x0, y0)
+    ref = oracle_rays(oracle_bhm, cfg, px, py)
+    assert_rays_equal(gpu, ref, "c2 block")
+    assert (gpu["I"][..., 0] > 0).mean() > 0.5
+    rng = np.random.default_rng(3)
+    sx, sy = rng.integers(0, 1024, 1024), rng.integers(0, 1024, 1024)
+    full = t.trace_tile(cfg["F"], 0, 0, 1024, 1024)
+    sel = full[sy, sx]
+    ref = oracle_rays(oracle_bhm, cfg, sx.astype(np.int32), sy.astype(np.int32))
+    assert_rays_equal(sel, ref, "c2 scattered")
+
+
+def test_radio_frame_flux_256(oracle_bhm):
+    """Frame interface: a 256^2 radio frame; per-pixel float4 equal to the oracle's FP64 sums cast
+    to float, and the frame flux equal to 1e-12 relative (different summation order)."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=256, H=256)
+    t = tracer_for(cfg)
+    img, st = t.render(cfg["F"])
+    px, py = pixel_grid(256, 256)
+    out_px, rays, _ = oracle_bhm.trace_pixels(cfg["OS"], cfg["OF"], px, py, spp=1, jitter=False)
+    assert np.array_equal(img.reshape(-1, 4), out_px.astype(np.float32))
+    flux_ref = rays["I"][:, 0].sum()
+    assert abs(st.flux_sum - flux_ref) <= 1e-12 * abs(flux_ref)
+    assert st.step_attempts == rays["steps"].sum()
+    assert st.flow_steps == rays["flow_steps"].sum()
+    assert st.step_limited == (rays["fate"] == 0).sum()
+
+
+def test_radio_flux_jittered(oracle_bhm):
+    """radio_flux (n^2, spp 2, R2 + PCG jitter) equals the oracle's jittered sum."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=48, H=48)
+    t = tracer_for(cfg)
+    F = cfg["F"]
+    F.seed = 1
+    pix_sr = (40.0 / 48 * cfg["info"]["rg_cm"] / cfg["info"]["distance_cm"]) ** 2
+    jy = t.radio_flux(F, 48, pix_sr)
+    OF = cfg["OF"]
+    OF.seed = 1
+    OF.spp = 2
+    px, py = pixel_grid(48, 48)
+    out_px, _, _ = oracle_bhm.trace_pixels(cfg["OS"], OF, px, py, spp=2, jitter=True)
+    want = out_px[:, 0].sum() * pix_sr / 1e-23
+    assert abs(jy - want) <= 1e-12 * abs(want)
+
+
+def test_vacuum_fates_match_reference(golden):
+    """bh_vacuum_fates on the reference-generated photons reproduces the reference's fates."""
+    S = bh.Scene()
+    S.a = 0.94
+    S.rh = 1 + math.sqrt(1 - 0.94 ** 2)
+    S.tol = 1e-10
+    S.max_steps = 20000
+    t = bh.Tracer(S)
+    fates = t.vacuum_fates(golden["fate_photons"], 1e-10, 1100.0, 20000)
+    assert np.array_equal(fates, golden["fate_tol1e-10"])
+
+
+def test_selftest_on_gpu():
+    rows = bh.selftest(0)
+    lines = ["a=%.2f incl=%4.0f  ray-traced edges %+.6f %+.6f   analytic %+.6f %+.6f  (M)" % tuple(r) for r in rows]
+    import os
+    from helpers import ROOT
+    want = [l.strip() for l in open(os.path.join(ROOT, "tests", "golden", "selftest_reference.txt"))
+            if l.startswith("a=")]
+    assert lines == want
+
+
+def test_zoom_target_on_gpu(golden):
+    S = bh.Scene()
+    S.a = 0.94
+    S.rh = 1 + math.sqrt(1 - 0.94 ** 2)
+    S.tol = 1e-12
+    S.max_steps = 2000000
+    t = bh.Tracer(S)
+    F = bh.make_frame(0.94, 1000.0, 17 * D2R, fov=30.0, W=1)
+    rho = t.find_critical_point(F, -100 * D2R)
+    assert abs(rho - golden["fcp_out"][-2]) < 1e-9
+
+
+def test_error_paths():
+    cfg = setup(shot("c1_vacuum_256.ini"))
+    t = tracer_for(cfg)
+    F = cfg["F"]
+    bad = bh.Frame.from_buffer_copy(F)
+    bad.pix = 0.0
+    with pytest.raises(bh.BhError):
+        t.render(bad)
+    with pytest.raises(bh.BhError):
+        t.trace_tile(F, 0, 0, 0, 4)
+    with pytest.raises(bh.BhError):
+        t.render(F, bh.Tracer.opts(precision=7))
+    # radio without the K2 table is refused, not traced
+    S = cfg["S"]
+    S2 = bh.Scene.from_buffer_copy(S)
+    S2.mode = bh.MODE_RADIO
+    S2.flow_on = 1
+    t2 = bh.Tracer(S2)
+    with pytest.raises(bh.BhError):
+        t2.render(F)
