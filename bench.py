@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-pixels", type=int, default=24576)
+    ap.add_argument("--cpu-pixels", type=int, default=1048576)
     ap.add_argument("--modes", default="fp64,mixed")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -129,13 +129,25 @@ def main():
     modes = [m for m in args.modes.split(",") if m]
     for m in modes:
         o = bh.Tracer.opts(precision=bh.PREC_FP64 if m == "fp64" else bh.PREC_MIXED,
-                           emissivity=bh.EMIS_EXACT if m == "fp64" else bh.EMIS_MLP, jitter=0, spp=1)
+                           emissivity=bh.EMIS_MLP if m == "mixed_mlp" else bh.EMIS_EXACT, jitter=0, spp=1)
         try:
             results[m] = bench_mode(tracer, F, o, args.steps, args.warmup)
             log(f"bench: {m}: {results[m]['mrays_per_s']:.3f} Mrays/s, kernel {results[m]['kernel_ms']:.2f} ms, "
                 f"{results[m]['achieved_tflops']:.2f} TFLOP/s")
         except bh.BhError as e:
             log(f"bench: {m}: not available ({e})")
+
+    # accuracy of the mixed tracer against the FP64 tracer (bit-identical to the oracle) on this frame
+    pix_err = None
+    if "mixed" in results:
+        ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1))
+        mix, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_EXACT, jitter=0, spp=1))
+        Ia, Ib = mix[..., 0].astype(np.float64), ref[..., 0].astype(np.float64)
+        mask = Ib > 1e-3 * Ib.max()
+        rel = np.abs(Ia[mask] - Ib[mask]) / Ib[mask]
+        pix_err = {"flux_rel": abs(Ia.sum() - Ib.sum()) / Ib.sum(), "pixel_median_rel": float(np.median(rel)),
+                   "pixel_p99_rel": float(np.percentile(rel, 99)), "pixels": int(mask.sum())}
+        log(f"bench: mixed vs fp64 frame: {pix_err}")
 
     head = results.get("mixed") or results["fp64"]
     head_mode = "mixed" if "mixed" in results else "fp64"
@@ -184,6 +196,7 @@ def main():
         line["mixed"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed"].items()}
         line["speedup_mixed_vs_fp64"] = round(results["mixed"]["mrays_per_s"] / fp64["mrays_per_s"], 3)
         line["flux_rel_err_mixed_vs_fp64"] = flux_err
+        line["mixed_vs_fp64_frame_error"] = pix_err
     if cpu:
         line["gpu_vs_cpu"] = round(head["mrays_per_s"] / cpu["value"], 1)
     tracer.close()

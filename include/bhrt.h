@@ -85,6 +85,8 @@ typedef struct bh_trace_opts {
     int emissivity;   /* BH_EMIS_* (radio only) */
     int jitter;       /* 0: pixel-centre samples; 1: R2 + PCG jitter as render.cu:70-85 */
     int spp;          /* samples per pixel; 0 -> frame->spp */
+    double tol_far;   /* mixed: tolerance of the FP32 far-field steps (0 -> 1e-6) */
+    double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */
@@ -153,6 +155,51 @@ int bh_radio_flux(bh_ctx* ctx, const bh_frame* frame, int n, double pix_sr, doub
    (shot.cpp:148-164) and the self-test. */
 int bh_vacuum_fates(bh_ctx* ctx, const double* photons, int n, double tol, double r_escape, int max_steps,
                     int* fates);
+
+/* ---- Thermal-synchrotron surrogate (new: the reference has only the exact formula,
+   blackholes/src/emission.cuh:159-170). A 3 -> 64 -> 64 -> 64 -> 2 SiLU MLP of
+   (log10 nu/nu_c, Theta_e, theta_B) -> (ln f_j, ln f_a), f_j = j / (n_e e^2 nu_c / c),
+   f_a = alpha / (n_e e^2 / (m_e c nu_c)), trained on the GPU on samples of the exact routine. ---- */
+typedef struct bh_mlp_train_cfg {
+    int n_samples;      /* training samples (0 -> 2^22) */
+    int batch;          /* samples per Adam step, multiple of 16 (0 -> 8192) */
+    int epochs;         /* passes over the samples (0 -> 40) */
+    float lr, lr_min;   /* cosine-decayed Adam learning rate (0 -> 2e-3, 2e-5) */
+    float hot_weight;   /* extra loss weight where ln f_j > -12 (< 0 -> 20) */
+    uint64_t seed;      /* sampler and initialisation seed (0 -> 1) */
+} bh_mlp_train_cfg;
+
+typedef struct bh_mlp_report {
+    double ms_sample, ms_train;
+    int steps;
+    float loss_first, loss_last;
+    double train_flop;
+} bh_mlp_report;
+
+typedef struct bh_mlp_bake_report {
+    double ms_bake, ms_judge;
+    long long cells, hot_cells, failed_hot_cells;
+} bh_mlp_bake_report;
+
+/* Samples the exact routine over the box and trains the surrogate on the context's device. */
+int bh_mlp_train(bh_ctx* ctx, const bh_mlp_train_cfg* cfg, bh_mlp_report* report);
+/* Parameters (8706 floats: W1[3][64] b1 W2[64][64] b2 W3[64][64] b3 W4[64][2] b4) and the
+   normalisation (6 floats: log10-Theta centre, half-width, output means[2], scales[2]). */
+int bh_mlp_get(bh_ctx* ctx, float* params, float* norm);
+int bh_mlp_set(bh_ctx* ctx, const float* params, const float* norm);
+/* Inference kernel on n raw inputs (log10 nu/nu_c, Theta_e, theta_B) -> (ln f_j, ln f_a). */
+int bh_mlp_infer(bh_ctx* ctx, const float* in, int n, float* out);
+/* The exact routine (FP64) at the same inputs -> (ln f_j, ln f_a). */
+int bh_mlp_exact(bh_ctx* ctx, const float* in, int n, double* out);
+/* Loss and gradient of the training kernels for given parameters, normalisation, normalised inputs
+   xn[n][3], raw targets tn[n][2] and weights w[n] (n a multiple of 16). */
+int bh_mlp_grad(bh_ctx* ctx, const float* params, const float* norm, const float* xn, const float* tn, const float* w,
+                int n, float* loss, float* grad);
+/* Tabulates the trained network on an n0 x n1 x n2 lattice over (log10 nu/nu_c, log10 Theta,
+   theta_B in [0, pi/2]) with the inference kernel and judges every cell against the exact routine
+   (|d ln f| > judge_tol fails it). The mixed tracer with BH_EMIS_MLP reads this table and uses the
+   exact formula in failed cells and outside the box. */
+int bh_mlp_bake(bh_ctx* ctx, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_report* report);
 
 /* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context
    needed; device 0). fn: 0 exp, 1 log, 2 expm1, 3 cbrt, 4 x^-0.2, 5 sin, 6 cos, 7 atan2(x, y),

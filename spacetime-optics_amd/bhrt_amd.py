@@ -22,7 +22,9 @@ EXPORTS = [
     "bh_trace_tile", "bh_radio_flux", "bh_vacuum_fates", "bh_math_eval", "bh_destroy", "bh_last_error",
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
+    "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
 ]
+MLP_NPARAM = 8706
 
 
 class Scene(C.Structure):
@@ -58,7 +60,8 @@ class Settings(C.Structure):
 
 
 class TraceOpts(C.Structure):
-    _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int)]
+    _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
+                ("tol_far", C.c_double), ("r_near", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -67,6 +70,27 @@ class Stats(C.Structure):
         ("step_limited", C.c_longlong), ("step_attempts", C.c_longlong), ("steps_accepted", C.c_longlong),
         ("flow_steps", C.c_longlong), ("fp64_steps", C.c_longlong), ("flux_sum", C.c_double),
     ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class MlpTrainCfg(C.Structure):
+    _fields_ = [("n_samples", C.c_int), ("batch", C.c_int), ("epochs", C.c_int), ("lr", C.c_float),
+                ("lr_min", C.c_float), ("hot_weight", C.c_float), ("seed", C.c_uint64)]
+
+
+class MlpReport(C.Structure):
+    _fields_ = [("ms_sample", C.c_double), ("ms_train", C.c_double), ("steps", C.c_int), ("loss_first", C.c_float),
+                ("loss_last", C.c_float), ("train_flop", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class MlpBakeReport(C.Structure):
+    _fields_ = [("ms_bake", C.c_double), ("ms_judge", C.c_double), ("cells", C.c_longlong),
+                ("hot_cells", C.c_longlong), ("failed_hot_cells", C.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -111,6 +135,20 @@ def load():
     L.bh_radio_flux.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_double, _D]
     L.bh_vacuum_fates.restype = C.c_int
     L.bh_vacuum_fates.argtypes = [C.c_void_p, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.bh_mlp_train.restype = C.c_int
+    L.bh_mlp_train.argtypes = [C.c_void_p, C.POINTER(MlpTrainCfg), C.POINTER(MlpReport)]
+    L.bh_mlp_get.restype = C.c_int
+    L.bh_mlp_get.argtypes = [C.c_void_p, _F, _F]
+    L.bh_mlp_set.restype = C.c_int
+    L.bh_mlp_set.argtypes = [C.c_void_p, _F, _F]
+    L.bh_mlp_infer.restype = C.c_int
+    L.bh_mlp_infer.argtypes = [C.c_void_p, _F, C.c_int, _F]
+    L.bh_mlp_exact.restype = C.c_int
+    L.bh_mlp_exact.argtypes = [C.c_void_p, _F, C.c_int, _D]
+    L.bh_mlp_grad.restype = C.c_int
+    L.bh_mlp_grad.argtypes = [C.c_void_p, _F, _F, _F, _F, _F, C.c_int, _F, _F]
+    L.bh_mlp_bake.restype = C.c_int
+    L.bh_mlp_bake.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(MlpBakeReport)]
     L.bh_math_eval.restype = C.c_int
     L.bh_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
     L.bh_destroy.argtypes = [C.c_void_p]
@@ -301,8 +339,8 @@ class Tracer:
         return S
 
     @staticmethod
-    def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1):
-        return TraceOpts(precision, emissivity, jitter, spp)
+    def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0):
+        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)
@@ -338,6 +376,56 @@ class Tracer:
         self._check(self.L.bh_vacuum_fates(self.ctx, ph.ctypes.data_as(_D), len(ph), tol, r_escape, max_steps,
                                            fates.ctypes.data_as(C.POINTER(C.c_int))), "bh_vacuum_fates")
         return fates
+
+    # --- surrogate ----------------------------------------------------------------------------
+    def mlp_train(self, n_samples=0, batch=0, epochs=0, lr=0.0, lr_min=0.0, hot_weight=-1.0, seed=0):
+        cfg = MlpTrainCfg(n_samples, batch, epochs, lr, lr_min, hot_weight, seed)
+        rep = MlpReport()
+        self._check(self.L.bh_mlp_train(self.ctx, C.byref(cfg), C.byref(rep)), "bh_mlp_train")
+        return rep.as_dict()
+
+    def mlp_get(self):
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        nrm = np.zeros(6, dtype=np.float32)
+        self._check(self.L.bh_mlp_get(self.ctx, p.ctypes.data_as(_F), nrm.ctypes.data_as(_F)), "bh_mlp_get")
+        return p, nrm
+
+    def mlp_set(self, params, norm):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nrm = np.ascontiguousarray(norm, dtype=np.float32)
+        self._check(self.L.bh_mlp_set(self.ctx, p.ctypes.data_as(_F), nrm.ctypes.data_as(_F)), "bh_mlp_set")
+
+    def mlp_infer(self, u0, theta, theta_b):
+        x = np.ascontiguousarray(np.column_stack([u0, theta, theta_b]), dtype=np.float32)
+        out = np.zeros((len(x), 2), dtype=np.float32)
+        self._check(self.L.bh_mlp_infer(self.ctx, x.ctypes.data_as(_F), len(x), out.ctypes.data_as(_F)),
+                    "bh_mlp_infer")
+        return out
+
+    def mlp_exact(self, u0, theta, theta_b):
+        x = np.ascontiguousarray(np.column_stack([u0, theta, theta_b]), dtype=np.float32)
+        out = np.zeros((len(x), 2), dtype=np.float64)
+        self._check(self.L.bh_mlp_exact(self.ctx, x.ctypes.data_as(_F), len(x), out.ctypes.data_as(_D)),
+                    "bh_mlp_exact")
+        return out
+
+    def mlp_grad(self, params, norm, xn, tn, w):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nrm = np.ascontiguousarray(norm, dtype=np.float32)
+        xn = np.ascontiguousarray(xn, dtype=np.float32)
+        tn = np.ascontiguousarray(tn, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.L.bh_mlp_grad(self.ctx, p.ctypes.data_as(_F), nrm.ctypes.data_as(_F), xn.ctypes.data_as(_F),
+                                       tn.ctypes.data_as(_F), w.ctypes.data_as(_F), len(xn), C.byref(loss),
+                                       g.ctypes.data_as(_F)), "bh_mlp_grad")
+        return loss.value, g
+
+    def mlp_bake(self, n0=0, n1=0, n2=0, judge_tol=0.0):
+        rep = MlpBakeReport()
+        self._check(self.L.bh_mlp_bake(self.ctx, n0, n1, n2, judge_tol, C.byref(rep)), "bh_mlp_bake")
+        return rep.as_dict()
 
     def find_critical_point(self, F, psi):
         rho = C.c_double()

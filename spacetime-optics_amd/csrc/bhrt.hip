@@ -10,7 +10,9 @@
    order, so frame totals are deterministic (no floating-point atomics). */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -18,6 +20,8 @@
 
 #include "../../include/bhrt.h"
 #include "trace_fp64.h"
+#include "trace_mixed.h"
+#include "ctx.h"
 
 using bhd::RayResult;
 using bhd::TraceParams;
@@ -27,12 +31,8 @@ namespace {
 constexpr int TILE = 16;          // pixels per tile side
 constexpr int BLOCK = TILE * TILE;
 constexpr int NXCD = 8;
-constexpr int NCNT = 5;           // counters per block: attempts, accepted, flow, limited, fp64 attempts
 
-struct BlockOut {
-    double flux;
-    unsigned long long cnt[NCNT];
-};
+
 
 // tile index of this workgroup after the XCD-aware renumbering; -1 for padding workgroups
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles, int per_xcd) {
@@ -50,9 +50,9 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 template <int PREC, bool RADIO, bool RAYS>
-__global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, float4* __restrict__ px_out,
-                                                 bh_ray_out* __restrict__ rays_out, BlockOut* __restrict__ blk,
-                                                 int per_xcd) {
+__global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, const bhm32::MixedParams M,
+                                                 float4* __restrict__ px_out, bh_ray_out* __restrict__ rays_out,
+                                                 BlockOut* __restrict__ blk, int per_xcd) {
     const int tiles_x = (P.w + TILE - 1) / TILE, tiles_y = (P.h + TILE - 1) / TILE;
     const int tile = xcd_tile(blockIdx.x, tiles_x * tiles_y, per_xcd);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -68,8 +68,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, float4* __
             for (int s = 0; s < P.spp; ++s) {
                 double ox, oy;
                 bhd::sample_offsets(P, px, py, s, &ox, &oy);
-                bhd::trace_ray_fp64<RADIO>(P, ox, oy, r);
-                const int fp64_attempts = r.steps;
+                int fp64_attempts;
+                if (PREC == BH_PREC_FP64) {
+                    bhd::trace_ray_fp64<RADIO>(P, ox, oy, r);
+                    fp64_attempts = r.steps;
+                } else {
+                    fp64_attempts = bhm32::trace_ray_mixed<RADIO>(P, M, ox, oy, r);
+                }
                 a0 += r.I[0];
                 a1 += r.I[1];
                 a2 += r.I[2];
@@ -187,30 +192,6 @@ std::string g_create_error;
 
 }  // namespace
 
-struct bh_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bh_scene S{};
-    bh_settings rs{};
-    float* d_k2 = nullptr;
-    int k2_n = 0;
-    float k2_l0 = 0, k2_dl = 0;
-    float* d_turb = nullptr;
-    int turb_n = 0, turb_nt = 0;
-    double turb_half = 0, turb_t0 = 0, turb_dt = 0;
-    float4* d_px = nullptr;
-    size_t px_cap = 0;
-    bh_ray_out* d_rays = nullptr;
-    size_t rays_cap = 0;
-    BlockOut* d_blk = nullptr;
-    size_t blk_cap = 0;
-    BlockOut* d_tot = nullptr;
-    double* d_ph = nullptr;
-    int* d_fates = nullptr;
-    size_t ph_cap = 0;
-    std::string err;
-};
 
 #define HIPCK(ctx, x)                                                                                      \
     do {                                                                                                   \
@@ -308,14 +289,49 @@ int check_frame(bh_ctx* c, const bh_frame* f) {
     return BH_OK;
 }
 
-using KernelFn = void (*)(const TraceParams, float4*, bh_ray_out*, BlockOut*, int);
+using KernelFn = void (*)(const TraceParams, const bhm32::MixedParams, float4*, bh_ray_out*, BlockOut*, int);
 
 KernelFn pick_kernel(int prec, bool radio, bool rays) {
     if (prec == BH_PREC_FP64) {
         if (radio) return rays ? k_trace<BH_PREC_FP64, true, true> : k_trace<BH_PREC_FP64, true, false>;
         return rays ? k_trace<BH_PREC_FP64, false, true> : k_trace<BH_PREC_FP64, false, false>;
     }
-    return nullptr;
+    if (radio) return rays ? k_trace<BH_PREC_MIXED, true, true> : k_trace<BH_PREC_MIXED, true, false>;
+    return rays ? k_trace<BH_PREC_MIXED, false, true> : k_trace<BH_PREC_MIXED, false, false>;
+}
+
+// Log-space emission constants and the region split of the mixed policy
+bhm32::MixedParams make_mixed(const bh_scene& S, const bh_trace_opts& o) {
+    constexpr double C = 2.99792458e10, H = 6.62607015e-27, ME = 9.1093837e-28, MP = 1.67262192e-24,
+                     QE = 4.80320471e-10, PI = 3.14159265358979323846;
+    bhm32::MixedParams M;
+    std::memset(&M, 0, sizeof M);
+    M.tol_far = o.tol_far > 0 ? o.tol_far : std::max(1e-6, S.tol);
+    const double r_ph = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::min(1.0, std::fabs(S.a)))));
+    const double r_near = o.r_near > 0 ? o.r_near : std::max(3.0 * S.rh, r_ph) + 1.0;
+    M.r_far = (S.mode == BH_MODE_RADIO && S.flow_on) ? std::max(S.r_flow_max, r_near) : r_near;
+    bhm32::EmisParams& E = M.E;
+    E.ln_n0 = (float)std::log(S.n0);
+    E.n_index = (float)S.n_index;
+    E.ln_theta0 = (float)std::log(S.theta0);
+    E.theta_index = (float)S.theta_index;
+    E.h_r = (float)S.h_r;
+    E.r_flow_max = (float)S.r_flow_max;
+    E.rh = (float)S.rh;
+    E.r_isco = (float)S.r_isco;
+    E.flow_kep = (float)S.flow_kep;
+    E.a = (float)S.a;
+    E.a2 = (float)(S.a * S.a);
+    E.ln_bconst = (float)std::log(8.0 * PI * MP * C * C / (6.0 * S.beta_p));
+    E.ln_nuc_per_B = (float)std::log(QE / (2.0 * PI * ME * C));
+    E.ln_jconst = (float)std::log(1.4142135623730951 * PI * QE * QE / (3.0 * C));
+    E.ln_bnu_const = (float)std::log(2.0 * H / (C * C));
+    E.h_over_mec2 = (float)(H / (ME * C * C));
+    E.ln_nu_obs = (float)std::log(S.nu_obs);
+    E.ln_rg = (float)std::log(S.rg_cm);
+    M.ln_e2_over_c = (float)std::log(QE * QE / C);
+    M.ln_e2_over_mec = (float)std::log(QE * QE / (ME * C));
+    return M;
 }
 
 // Launches the trace of a w x h tile at (x0, y0) and the block reduction; fills stats counters.
@@ -327,8 +343,16 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     bh_trace_opts o{};
     if (opts) o = *opts;
     if (o.precision != BH_PREC_FP64 && o.precision != BH_PREC_MIXED) return fail(c, BH_ERR_ARG, "bad precision");
-    if (o.precision == BH_PREC_MIXED) return fail(c, BH_ERR_UNSUPPORTED, "mixed precision is not built yet");
-    if (o.emissivity != BH_EMIS_EXACT) return fail(c, BH_ERR_UNSUPPORTED, "MLP emissivity is not built yet");
+    if (o.emissivity != BH_EMIS_EXACT && o.emissivity != BH_EMIS_MLP) return fail(c, BH_ERR_ARG, "bad emissivity");
+    if (o.emissivity == BH_EMIS_MLP && o.precision != BH_PREC_MIXED)
+        return fail(c, BH_ERR_ARG, "the MLP emissivity runs in the mixed-precision tracer");
+    if (o.emissivity == BH_EMIS_MLP && !c->stab_ready)
+        return fail(c, BH_ERR_UNSUPPORTED, "no baked surrogate: call bh_mlp_train (or bh_mlp_set) and bh_mlp_bake first");
+    bhm32::MixedParams M = make_mixed(c->S, o);
+    if (o.emissivity == BH_EMIS_MLP) {
+        M.use_surrogate = 1;
+        M.S = c->stab;
+    }
     TraceParams P = make_params(c, f);
     P.x0 = x0;
     P.y0 = y0;
@@ -347,7 +371,7 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     if (rc) return rc;
     KernelFn k = pick_kernel(o.precision, c->S.mode == BH_MODE_RADIO, rays);
     HIPCK(c, hipEventRecord(c->ev0, c->st));
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, c->d_px, c->d_rays, c->d_blk, per_xcd);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, M, c->d_px, c->d_rays, c->d_blk, per_xcd);
     HIPCK(c, hipGetLastError());
     HIPCK(c, hipEventRecord(c->ev1, c->st));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, nb, c->d_tot);
@@ -586,7 +610,8 @@ void bh_destroy(bh_ctx* c) {
     // best-effort teardown: errors here have nowhere to go
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* bufs[] = {c->d_k2, c->d_turb, c->d_px, c->d_rays, c->d_blk, c->d_tot, c->d_ph, c->d_fates};
+    void* bufs[] = {c->d_k2,  c->d_turb, c->d_px,     c->d_rays,     c->d_blk,     c->d_tot,
+                    c->d_ph,  c->d_fates, c->d_mlp, c->d_adam_m, c->d_adam_v, c->d_stab, c->d_sbad};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -1,0 +1,51 @@
+/* ctx.h - the device context behind the C ABI (host-side definition shared by bhrt.hip and mlp.hip). */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/bhrt.h"
+#include "mlp.h"
+
+constexpr int NCNT = 5;   // counters per block: attempts, accepted, flow, limited, fp64 attempts
+
+struct BlockOut {
+    double flux;
+    unsigned long long cnt[NCNT];
+};
+
+
+struct bh_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bh_scene S{};
+    bh_settings rs{};
+    float* d_k2 = nullptr;
+    int k2_n = 0;
+    float k2_l0 = 0, k2_dl = 0;
+    float* d_turb = nullptr;
+    int turb_n = 0, turb_nt = 0;
+    double turb_half = 0, turb_t0 = 0, turb_dt = 0;
+    float4* d_px = nullptr;
+    size_t px_cap = 0;
+    bh_ray_out* d_rays = nullptr;
+    size_t rays_cap = 0;
+    BlockOut* d_blk = nullptr;
+    size_t blk_cap = 0;
+    BlockOut* d_tot = nullptr;
+    double* d_ph = nullptr;
+    int* d_fates = nullptr;
+    size_t ph_cap = 0;
+    // surrogate: FP32 master parameters, Adam moments, normalisation and the baked table
+    float* d_mlp = nullptr;
+    float* d_adam_m = nullptr;
+    float* d_adam_v = nullptr;
+    bhmlp::Norm mlp_norm{};
+    bool mlp_trained = false;
+    float2* d_stab = nullptr;
+    unsigned char* d_sbad = nullptr;
+    bhmlp::SurrogateTable stab{};
+    bool stab_ready = false;
+    std::string err;
+};

@@ -1,0 +1,440 @@
+/* trace_mixed.h - mixed FP32/FP64 per-ray tracer for gfx950 (device code, included by bhrt.hip).
+
+   Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame):
+   - Far field (r >= r_far: outside the emitting flow in radio mode, outside r_near in vacuum mode):
+     DP5(4) stages in FP32 (FP32 reciprocals instead of divisions), the state y accumulated in FP64,
+     the error norm, step controller and radial-constraint projection in FP64, tolerance tol_far.
+     The projection r' = +-sqrt(R(r)) after every accepted step (FP64) keeps the radial motion on
+     the photon's constant-of-motion surface, so FP32 stage errors only shift where along the
+     trajectory a step lands.
+   - Inside the flow and near the hole: FP64 geodesic at the scene tolerance (inside the flow the
+     reference's steps are tolerance-limited, and the radiative-transfer quadrature depends on that
+     step sequence), evaluated with fused multiply-adds and Newton-refined FP64 reciprocals.
+   - Emission and absorption in FP32, in log space (the cgs coefficients span 1e-45..1e-10), exact
+     thermal-synchrotron formula or the MLP surrogate table; intensity and transmission accumulate
+     in FP64. The swept angle (photon-ring order) is tracked in FP32.
+   None of this is bit-compatible with the FP64 tracer; its error against the FP64 oracle is what
+   the tests bound. */
+#pragma once
+#include "mlp.h"
+#include "trace_fp64.h"
+
+namespace bhm32 {
+
+using bhd::RayConst;
+using bhd::RayResult;
+using bhd::State;
+using bhd::TraceParams;
+
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpd(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <typename R>
+__device__ __forceinline__ R rcp(R x);
+template <>
+__device__ __forceinline__ float rcp<float>(float x) { return rcpf(x); }
+template <>
+__device__ __forceinline__ double rcp<double>(double x) { return rcpd(x); }
+
+// natural log / exp on the hardware base-2 units
+__device__ __forceinline__ float lnf(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+__device__ __forceinline__ float expf_(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+
+// x^-0.2 to FP32 accuracy (step-size controller factor), x > 0
+__device__ __forceinline__ double pow_m02_fast(double x) {
+    float f = (float)fmin(fmax(x, 1e-30), 1e30);
+    return (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(f));
+}
+
+template <typename R>
+struct Rhs {
+    R d1, d3, d4, d5;
+};
+
+template <typename R>
+struct RConst {
+    R a, a2, lam, eta, K, lam2, alam, s2min;
+};
+
+// blackholes/src/kerr.hpp:103-116 with reciprocals in place of the three divisions
+template <typename R>
+__device__ __forceinline__ Rhs<R> rhs(const RConst<R>& c, R r, R u) {
+    R r2 = r * r;
+    R Delta = r2 - R(2) * r + c.a2;
+    R Pp = r2 + c.a2 - c.alam;
+    R s2 = fmax(R(1) - u * u, c.s2min);
+    R iD = rcp<R>(Delta), is2 = rcp<R>(s2);
+    Rhs<R> o;
+    o.d1 = R(2) * r * Pp - (r - R(1)) * c.K;
+    o.d3 = u * (c.a2 * (R(1) - R(2) * u * u) - c.eta - c.lam2);
+    o.d4 = -(c.a * Pp * iD + c.lam * is2 - c.a);
+    o.d5 = -((r2 + c.a2) * Pp * iD + c.a * (c.lam - c.a * s2));
+    return o;
+}
+
+/* One Dormand-Prince 5(4) step with stage arithmetic in R (float: far field, double: near field),
+   the 5th-order state accumulated in FP64 and the scaled error norm in FP64. */
+template <typename R>
+__device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, double h, double tol, double tol4,
+                                       State& o) {
+    using namespace bhd;
+    const R hr = (R)h;
+    const R y0 = (R)s.y0, y1 = (R)s.y1, y2 = (R)s.y2, y3 = (R)s.y3;
+    R q0 = y1, q1 = (R)s.k1, q2 = y3, q3 = (R)s.k3;
+    R S3_0 = R(a31) * q0, S3_1 = R(a31) * q1, S3_2 = R(a31) * q2, S3_3 = R(a31) * q3;
+    R S4_0 = R(a41) * q0, S4_1 = R(a41) * q1, S4_2 = R(a41) * q2, S4_3 = R(a41) * q3;
+    R S5_0 = R(a51) * q0, S5_1 = R(a51) * q1, S5_2 = R(a51) * q2, S5_3 = R(a51) * q3;
+    R S6_0 = R(a61) * q0, S6_1 = R(a61) * q1, S6_2 = R(a61) * q2, S6_3 = R(a61) * q3;
+    R O0 = R(b1) * q0, O1 = R(b1) * q1, O2 = R(b1) * q2, O3 = R(b1) * q3, O4 = R(b1) * (R)s.k4,
+      O5 = R(b1) * (R)s.k5;
+    R E0 = R(e1) * q0, E1 = R(e1) * q1, E2 = R(e1) * q2, E3 = R(e1) * q3, E4 = R(e1) * (R)s.k4;
+    const R ha = hr * R(a21);
+    R t0 = y0 + ha * q0, t1 = y1 + ha * q1, t2 = y2 + ha * q2, t3 = y3 + ha * q3;
+    Rhs<R> k = rhs<R>(c, t0, t2);
+    q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
+    S3_0 += R(a32) * q0; S3_1 += R(a32) * q1; S3_2 += R(a32) * q2; S3_3 += R(a32) * q3;
+    S4_0 += R(a42) * q0; S4_1 += R(a42) * q1; S4_2 += R(a42) * q2; S4_3 += R(a42) * q3;
+    S5_0 += R(a52) * q0; S5_1 += R(a52) * q1; S5_2 += R(a52) * q2; S5_3 += R(a52) * q3;
+    S6_0 += R(a62) * q0; S6_1 += R(a62) * q1; S6_2 += R(a62) * q2; S6_3 += R(a62) * q3;
+    t0 = y0 + hr * S3_0; t1 = y1 + hr * S3_1; t2 = y2 + hr * S3_2; t3 = y3 + hr * S3_3;
+    k = rhs<R>(c, t0, t2);
+    q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
+    S4_0 += R(a43) * q0; S4_1 += R(a43) * q1; S4_2 += R(a43) * q2; S4_3 += R(a43) * q3;
+    S5_0 += R(a53) * q0; S5_1 += R(a53) * q1; S5_2 += R(a53) * q2; S5_3 += R(a53) * q3;
+    S6_0 += R(a63) * q0; S6_1 += R(a63) * q1; S6_2 += R(a63) * q2; S6_3 += R(a63) * q3;
+    O0 += R(b3) * q0; O1 += R(b3) * q1; O2 += R(b3) * q2; O3 += R(b3) * q3; O4 += R(b3) * k.d4; O5 += R(b3) * k.d5;
+    E0 += R(e3) * q0; E1 += R(e3) * q1; E2 += R(e3) * q2; E3 += R(e3) * q3; E4 += R(e3) * k.d4;
+    t0 = y0 + hr * S4_0; t1 = y1 + hr * S4_1; t2 = y2 + hr * S4_2; t3 = y3 + hr * S4_3;
+    k = rhs<R>(c, t0, t2);
+    q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
+    S5_0 += R(a54) * q0; S5_1 += R(a54) * q1; S5_2 += R(a54) * q2; S5_3 += R(a54) * q3;
+    S6_0 += R(a64) * q0; S6_1 += R(a64) * q1; S6_2 += R(a64) * q2; S6_3 += R(a64) * q3;
+    O0 += R(b4) * q0; O1 += R(b4) * q1; O2 += R(b4) * q2; O3 += R(b4) * q3; O4 += R(b4) * k.d4; O5 += R(b4) * k.d5;
+    E0 += R(e4) * q0; E1 += R(e4) * q1; E2 += R(e4) * q2; E3 += R(e4) * q3; E4 += R(e4) * k.d4;
+    t0 = y0 + hr * S5_0; t1 = y1 + hr * S5_1; t2 = y2 + hr * S5_2; t3 = y3 + hr * S5_3;
+    k = rhs<R>(c, t0, t2);
+    q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
+    S6_0 += R(a65) * q0; S6_1 += R(a65) * q1; S6_2 += R(a65) * q2; S6_3 += R(a65) * q3;
+    O0 += R(b5) * q0; O1 += R(b5) * q1; O2 += R(b5) * q2; O3 += R(b5) * q3; O4 += R(b5) * k.d4; O5 += R(b5) * k.d5;
+    E0 += R(e5) * q0; E1 += R(e5) * q1; E2 += R(e5) * q2; E3 += R(e5) * q3; E4 += R(e5) * k.d4;
+    t0 = y0 + hr * S6_0; t1 = y1 + hr * S6_1; t2 = y2 + hr * S6_2; t3 = y3 + hr * S6_3;
+    k = rhs<R>(c, t0, t2);
+    q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
+    O0 += R(b6) * q0; O1 += R(b6) * q1; O2 += R(b6) * q2; O3 += R(b6) * q3; O4 += R(b6) * k.d4; O5 += R(b6) * k.d5;
+    E0 += R(e6) * q0; E1 += R(e6) * q1; E2 += R(e6) * q2; E3 += R(e6) * q3; E4 += R(e6) * k.d4;
+    o.y0 = s.y0 + (double)(hr * O0);
+    o.y1 = s.y1 + (double)(hr * O1);
+    o.y2 = s.y2 + (double)(hr * O2);
+    o.y3 = s.y3 + (double)(hr * O3);
+    o.y4 = s.y4 + (double)(hr * O4);
+    o.y5 = s.y5 + (double)(hr * O5);
+    k = rhs<R>(c, (R)o.y0, (R)o.y2);
+    o.k1 = (double)k.d1;
+    o.k3 = (double)k.d3;
+    o.k4 = (double)k.d4;
+    o.k5 = (double)k.d5;
+    E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
+    // FP64 scaled max-norm (kerr.hpp:145-155)
+    const double sc0 = tol * (1e-6 + fmax(fabs(s.y0), fabs(o.y0)));
+    const double sc1 = tol * (1e-6 + fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
+    const double sc3 = tol * fmax(1.0, fmax(fabs(s.y3), fabs(o.y3)));
+    double m = fabs(h * (double)E0) * rcpd(sc0);
+    m = fmax(m, fabs(h * (double)E1) * rcpd(sc1));
+    m = fmax(m, fabs(h * (double)E2) * rcpd(tol));
+    m = fmax(m, fabs(h * (double)E3) * rcpd(sc3));
+    m = fmax(m, fabs(h * (double)E4) * rcpd(tol4));
+    return m;
+}
+
+template <typename R>
+__device__ __forceinline__ RConst<R> make_const(const TraceParams& P, const RayConst& c) {
+    RConst<R> k;
+    k.a = (R)P.a;
+    k.a2 = (R)P.a2;
+    k.lam = (R)c.lam;
+    k.eta = (R)c.eta;
+    k.K = (R)c.K;
+    k.lam2 = (R)c.lam2;
+    k.alam = (R)c.alam;
+    k.s2min = sizeof(R) == 4 ? (R)1e-30 : (R)1e-300;
+    return k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Emission (blackholes/src/emission.cuh:99-175) in FP32, log space.
+struct EmisParams {
+    // log-space constants, filled on the host
+    float ln_n0, n_index, ln_theta0, theta_index, h_r, r_flow_max, rh, r_isco, flow_kep, a, a2;
+    float ln_bconst;   // ln(8 pi m_p c^2 / (6 beta)) : B^2 = exp(ln_bconst) n_e / r
+    float ln_nuc_per_B;   // ln(e / (2 pi m_e c))
+    float ln_jconst;   // ln(sqrt(2) pi e^2 / (3 c))
+    float ln_bnu_const;   // ln(2 h / c^2)
+    float h_over_mec2;    // h / (m_e c^2)  (x = h nu / (k T_e) = h nu / (m_e c^2 Theta))
+    float ln_nu_obs;
+    float ln_rg;
+};
+
+// Thermal synchrotron: ln j and ln alpha from ln n_e, ln Theta, ln nu_f, ln nu_c, ln sin(theta_B)
+// (Leung et al. 2011 with the K2 table, Kirchhoff alpha = j / B_nu).
+__device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const EmisParams& E, float ln_ne,
+                                                  float ln_th, float ln_nu, float ln_nuc, float ln_sinb,
+                                                  float* ln_j, float* ln_a) {
+    const float ln_nus = -1.5040774f + ln_nuc + 2.f * ln_th + ln_sinb;   // ln(2/9) + ...
+    const float lnX = ln_nu - ln_nus;
+    const float X = expf_(lnX), x3 = expf_(lnX * (1.f / 3.f));
+    const float brk = __builtin_sqrtf(X) + 1.887748625363387f * __builtin_sqrtf(x3);
+    const float lnk2 = bhd::table_lerp(P.k2, P.k2_n, P.k2_l0, P.k2_dl, ln_th);
+    *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + 2.f * lnf(brk) - x3;
+    // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x))
+    const float x = E.h_over_mec2 * expf_(ln_nu - ln_th);
+    const float ln_em1 = x < 1e-3f ? lnf(x) + 0.5f * x : lnf(expf_(x) - 1.f);
+    const float ln_bnu = E.ln_bnu_const + 3.f * ln_nu - ln_em1;
+    *ln_a = *ln_j - ln_bnu;
+}
+
+// Geometry + plasma of one flow sample; returns false when nothing is emitted or absorbed.
+struct FlowSample {
+    float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl, ln_bnu;
+};
+
+__device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisParams& E, double rd, double ud,
+                                            double phid, double td, double lamd, double Ecamd, double hd,
+                                            FlowSample& f) {
+    const float r = (float)rd, u = (float)ud, lam = (float)lamd;
+    const float s2 = fmaxf(1.f - u * u, 1e-12f), s = __builtin_sqrtf(s2);
+    const float R = r * s, z = r * u;
+    if (r > E.r_flow_max || R < 1e-3f) return false;
+    const float zh = z * rcpf(E.h_r * R);
+    const float lnvert = -0.5f * zh * zh;
+    if (lnvert < -16.118096f) return false;   // vert < 1e-7
+    const float lr = lnf(r);
+    float ln_ne = E.ln_n0 - E.n_index * lr + lnvert;
+    if (P.turb_on) ln_ne += lnf((float)bhd::turb_factor(P, rd, ud, phid, td));
+    const float dr = r - E.rh;
+    const float soft = 1.f - expf_(-dr * dr * 4.f);
+    if (!(soft > 0.f)) return false;
+    ln_ne += lnf(soft);
+    if (ln_ne < -69.07755f) return false;   // n_e < 1e-30
+    const float ln_th = E.ln_theta0 - E.theta_index * lr;
+    const float ln_B = 0.5f * (E.ln_bconst + ln_ne - lr);
+    // fluid four-velocity (emission.cuh:138-149)
+    const float r2 = r * r, a = E.a, a2 = E.a2;
+    const float Sigma = r2 + a2 * u * u, Delta = r2 - 2.f * r + a2;
+    const float A = (r2 + a2) * (r2 + a2) - a2 * Delta * s2;
+    const float iS = rcpf(Sigma), iA = rcpf(A);
+    const float g_tt = -(1.f - 2.f * r * iS), g_tph = -2.f * a * r * s2 * iS, g_phph = A * s2 * iS;
+    const float omz = 2.f * a * r * iA;
+    const float Rk = fmaxf(R, E.r_isco);
+    float Om = E.flow_kep * rcpf(Rk * __builtin_sqrtf(Rk) + a);
+    const float vmax = __builtin_sqrtf(Sigma * Delta * iA) * __builtin_amdgcn_rsqf(g_phph) * 0.999f;
+    Om = omz + fmaxf(-vmax, fminf(vmax, Om - omz));
+    const float norm = -(g_tt + 2.f * g_tph * Om + g_phph * Om * Om);
+    if (!(norm > 0.f)) return false;
+    const float ut = __builtin_amdgcn_rsqf(norm);
+    const float kdotu = ut * (1.f - Om * lam);
+    if (!(kdotu > 0.f)) return false;
+    const float ln_kdotu = lnf(kdotu);
+    const float ln_nu = E.ln_nu_obs + lnf((float)Ecamd) + ln_kdotu;
+    const float u_ph = ut * (g_tph + g_phph * Om);
+    const float kb = lam - kdotu * u_ph;
+    const float bb = g_phph + u_ph * u_ph;
+    const float cosb = kb * rcpf(kdotu * __builtin_sqrtf(bb));
+    const float sinb2 = fmaxf(1.f - cosb * cosb, 1e-6f);
+    f.ln_ne = ln_ne;
+    f.ln_th = ln_th;
+    f.ln_nu = ln_nu;
+    f.ln_nuc = E.ln_nuc_per_B + ln_B;
+    f.ln_sinb = 0.5f * lnf(sinb2);
+    f.ln_g = E.ln_nu_obs - ln_nu;
+    f.ln_dl = ln_kdotu + lnf(Sigma) + lnf((float)hd) + E.ln_rg;
+    return true;
+}
+
+// dI = g^3 B_nu (1 - e^-tau), dtau = alpha dl (trace.cuh:131-143 with emission.cuh:167-174)
+__device__ __forceinline__ void transfer(const EmisParams& E, const FlowSample& f, float ln_j, float ln_a,
+                                         double* dI, double* dtau) {
+    const float tau = expf_(ln_a + f.ln_dl);
+    const float om = tau < 1e-4f ? tau * (1.f - 0.5f * tau) : 1.f - expf_(-tau);
+    const float ln_bnu = ln_j - ln_a;
+    *dtau = (double)tau;
+    *dI = (double)(expf_(3.f * f.ln_g + ln_bnu) * om);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Surrogate lookup: (ln f_j, ln f_a) of the tabulated network at (u0 = log10 nu/nu_c,
+// log10 Theta, theta_B folded into [0, pi/2]); false outside the box or in a cell the exact routine
+// judged inaccurate (the caller then evaluates the exact formula).
+__device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T, float u0, float lth10, float tb,
+                                                 float* fj, float* fa) {
+    const float g0 = (u0 - T.lo0) * T.inv_d0, g1 = (lth10 - T.lo1) * T.inv_d1, g2 = (tb - T.lo2) * T.inv_d2;
+    if (!(g0 >= 0.f && g1 >= 0.f && g2 >= 0.f && g0 <= (float)(T.n0 - 1) && g1 <= (float)(T.n1 - 1) &&
+          g2 <= (float)(T.n2 - 1)))
+        return false;
+    const int i0 = min((int)g0, T.n0 - 2), i1 = min((int)g1, T.n1 - 2), i2 = min((int)g2, T.n2 - 2);
+    if (T.bad[((size_t)i2 * (T.n1 - 1) + i1) * (T.n0 - 1) + i0]) return false;
+    const float f0 = g0 - i0, f1 = g1 - i1, f2 = g2 - i2;
+    const size_t s1 = (size_t)T.n0, s2 = (size_t)T.n0 * T.n1;
+    const float2* b = (const float2*)T.tab + (size_t)i2 * s2 + (size_t)i1 * s1 + i0;
+    const float2 v000 = b[0], v100 = b[1], v010 = b[s1], v110 = b[s1 + 1];
+    const float2 v001 = b[s2], v101 = b[s2 + 1], v011 = b[s2 + s1], v111 = b[s2 + s1 + 1];
+    float x00 = v000.x + (v100.x - v000.x) * f0, x10 = v010.x + (v110.x - v010.x) * f0;
+    float x01 = v001.x + (v101.x - v001.x) * f0, x11 = v011.x + (v111.x - v011.x) * f0;
+    float y00 = v000.y + (v100.y - v000.y) * f0, y10 = v010.y + (v110.y - v010.y) * f0;
+    float y01 = v001.y + (v101.y - v001.y) * f0, y11 = v011.y + (v111.y - v011.y) * f0;
+    float x0 = x00 + (x10 - x00) * f1, x1 = x01 + (x11 - x01) * f1;
+    float y0 = y00 + (y10 - y00) * f1, y1 = y01 + (y11 - y01) * f1;
+    *fj = x0 + (x1 - x0) * f2;
+    *fa = y0 + (y1 - y0) * f2;
+    return true;
+}
+
+struct MixedParams {
+    double r_far;     // far field (FP32 stages) at r >= r_far
+    double tol_far;   // tolerance of the far-field steps
+    EmisParams E;
+    int use_surrogate;
+    float ln_e2_over_c, ln_e2_over_mec;   // ln(e^2 / c), ln(e^2 / (m_e c))
+    bhmlp::SurrogateTable S;
+};
+
+/* blackholes/src/trace.cuh:62-165 under the mixed policy. Returns the number of FP64 step attempts
+   (near field and flow); out.steps counts all attempts. */
+template <bool RADIO>
+__device__ __forceinline__ int trace_ray_mixed(const TraceParams& P, const MixedParams& M, double x, double yv,
+                                               RayResult& out) {
+    RayConst c;
+    State s, n;
+    bhd::camera_photon(P, x, yv, c, s);
+    const RConst<float> cf = make_const<float>(P, c);
+    const RConst<double> cd = make_const<double>(P, c);
+    bool k1_f32 = s.y0 >= M.r_far;
+    if (k1_f32) {
+        Rhs<float> k = rhs<float>(cf, (float)s.y0, (float)s.y2);
+        s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
+    } else {
+        Rhs<double> k = rhs<double>(cd, s.y0, s.y2);
+        s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
+    }
+    double h = bhd::initial_step(s);
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0, trans = 1.0;
+    float psi = 0.f;
+    int ncross = 0, accepted = 0, flow_steps = 0, fate = 0, it = 0, fp64_attempts = 0;
+    double th_inf = 0.0, ph_inf = 0.0;
+    float ppx = 0.f, ppy = 0.f, ppz = 0.f;
+    const bool flow = RADIO && P.flow_on;
+    if (RADIO) {
+        float sp = __builtin_sqrtf(fmaxf(1.f - (float)(s.y2 * s.y2), 0.f));
+        float ph = (float)s.y4;
+        ppx = sp * __cosf(ph);
+        ppy = sp * __sinf(ph);
+        ppz = (float)s.y2;
+    }
+    for (it = 0; it < P.max_steps; ++it) {
+        double hc = h;
+        if (flow && s.y0 < P.r_flow_max) {
+            double sn = sqrt(fmax(1.0 - s.y2 * s.y2, 0.0));
+            double cap = P.step_frac * fmin(s.y0 * rcpd(fabs(s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(s.y3) + 1e-30));
+            hc = fmin(hc, cap);
+        }
+        double err;
+        if (s.y0 >= M.r_far) {
+            err = dp54<float>(cf, s, hc, M.tol_far, M.tol_far * 4.0, n);
+        } else {
+            if (k1_f32) {   // entering the FP64 region: FP64 derivative at the current state
+                Rhs<double> k = rhs<double>(cd, s.y0, s.y2);
+                s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
+            }
+            err = dp54<double>(cd, s, hc, P.tol, P.tol4, n);
+            ++fp64_attempts;
+        }
+        k1_f32 = s.y0 >= M.r_far;
+        if (!(err <= 1.0)) {
+            h = hc * fmax(0.2, 0.9 * pow_m02_fast(err));
+            if (!(err == err)) h = hc * 0.1;
+            continue;
+        }
+        ++accepted;
+        if (RADIO) {
+            float sn = __builtin_sqrtf(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
+            float ph = (float)n.y4;
+            float pnx = sn * __cosf(ph), pny = sn * __sinf(ph), pnz = (float)n.y2;
+            float cx = ppy * pnz - ppz * pny, cy = ppz * pnx - ppx * pnz, cz = ppx * pny - ppy * pnx;
+            psi += atan2f(__builtin_sqrtf(cx * cx + cy * cy + cz * cz), ppx * pnx + ppy * pny + ppz * pnz);
+            ppx = pnx;
+            ppy = pny;
+            ppz = pnz;
+        }
+        if ((s.y2 > 0.0) != (n.y2 > 0.0)) ++ncross;
+        if (flow && n.y0 < P.r_flow_max) {
+            ++flow_steps;
+            FlowSample f;
+            if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, c.lam, c.E, hc, f)) {
+                float ln_j, ln_a, fj, fa;
+                const float sinb = expf_(f.ln_sinb);
+                if (M.use_surrogate &&
+                    surrogate_lookup(M.S, (f.ln_nu - f.ln_nuc) * 0.43429448190325176f, f.ln_th * 0.43429448190325176f,
+                                     asinf(fminf(sinb, 1.f)), &fj, &fa)) {
+                    ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
+                    ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
+                } else {
+                    synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a);
+                }
+                double dI, dtau;
+                transfer(M.E, f, ln_j, ln_a, &dI, &dtau);
+                double cI = trans * dI;
+                int ord = (int)(psi * 0.31830988618379067f);
+                acc0 += cI;
+                if (ord == 1) acc1 += cI;
+                else if (ord == 2) acc2 += cI;
+                else if (ord >= 3) acc3 += cI;
+                trans *= (double)expf_(-(float)dtau);
+                if (trans < 1e-5) {
+                    fate = -1;
+                    break;
+                }
+            }
+        }
+        s = n;
+        bhd::project_radial(P, c, s);
+        h = hc * fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
+        if (s.y0 < P.r_stop) {
+            fate = -1;
+            break;
+        }
+        if (s.y0 > P.r_escape && s.y1 > 0.0) {
+            fate = 1;
+            double sq = sqrt(fmax(1.0 - s.y2 * s.y2, 1e-30));
+            double tq = (double)acosf(fmaxf(-1.f, fminf(1.f, (float)s.y2)));
+            // acos in FP32 is not enough near the poles: refine with one FP64 Newton step on cos
+            tq = tq + (cos(tq) - s.y2) * rcpd(fmax(sin(tq), 1e-30));
+            double dthdr = (-s.y3 * rcpd(sq)) * rcpd(s.y1), dphdr = s.k4 * rcpd(s.y1);
+            th_inf = tq + s.y0 * dthdr;
+            ph_inf = s.y4 + s.y0 * dphdr;
+            break;
+        }
+    }
+    out.y[0] = s.y0; out.y[1] = s.y1; out.y[2] = s.y2; out.y[3] = s.y3; out.y[4] = s.y4; out.y[5] = s.y5;
+    out.th_inf = th_inf;
+    out.ph_inf = ph_inf;
+    out.I[0] = acc0; out.I[1] = acc1; out.I[2] = acc2; out.I[3] = acc3;
+    out.trans = trans;
+    out.psi = (double)psi;
+    out.fate = fate;
+    out.ncross = ncross;
+    out.order = (int)(psi * 0.31830988618379067f);
+    out.steps = it < P.max_steps ? it + 1 : P.max_steps;
+    out.accepted = accepted;
+    out.flow_steps = flow_steps;
+    return fp64_attempts;
+}
+
+#pragma clang fp contract(off)
+
+}  // namespace bhm32

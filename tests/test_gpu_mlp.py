@@ -1,0 +1,122 @@
+"""Surrogate kernels on the GPU against the CPU oracle (oracle/mlp_ref.py), training on samples of the
+exact routine, and the trained network inside the mixed tracer."""
+import numpy as np
+import pytest
+
+import bhrt_amd as bh
+import mlp_ref as M
+from helpers import setup, shot, tracer_for
+
+pytestmark = pytest.mark.gpu
+
+N0_C2 = 291464.1043528089   # n0 of the 0.6 Jy normalisation of the 1024^2 frame
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    cfg = setup(shot("c2_radio_1024.ini"))
+    cfg["S"].n0 = N0_C2
+    t = tracer_for(cfg)
+    yield t, cfg
+    t.close()
+
+
+def random_params(seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=M.NPARAM) * scale
+    norm = np.array([0.8494850021680094, 1.1505149978319906, -6.0, -30.0, 8.0, 15.0], dtype=np.float32)
+    return p.astype(np.float32), norm
+
+
+def test_mfma_forward_matches_oracle(ctx):
+    t, _ = ctx
+    p, norm = random_params(1)
+    t.mlp_set(p, norm)
+    rng = np.random.default_rng(2)
+    n = 1000
+    u0, th, tb = rng.uniform(-4, 6, n), np.exp(rng.uniform(np.log(0.5), np.log(100), n)), rng.uniform(0, np.pi, n)
+    got = t.mlp_infer(u0, th, tb)
+    x32 = np.column_stack([u0, th, tb]).astype(np.float32).astype(np.float64)
+    y, _ = M.forward(p, M.normalise_inputs(x32[:, 0], x32[:, 1], x32[:, 2]))
+    want = y * norm[4:6] + norm[2:4]
+    err = np.abs(got - want) / norm[4:6]
+    print(f"forward: max normalised error {err.max():.2e}")
+    assert err.max() < 1e-4   # split-BF16 products keep ~16 significant bits
+
+
+def test_mfma_gradient_matches_oracle(ctx):
+    t, _ = ctx
+    p, norm = random_params(3)
+    rng = np.random.default_rng(4)
+    n = 256
+    xn = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    tr = (rng.normal(size=(n, 2)) * norm[4:6] + norm[2:4]).astype(np.float32)
+    w = rng.uniform(0.5, 3.0, n).astype(np.float32)
+    loss, g = t.mlp_grad(p, norm, xn, tr, w)
+    loss_ref, g_ref = M.loss_and_grad(p.astype(np.float64), norm, xn, tr, w)
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    for name, a, b in (("W1", M.OFF_W1, M.OFF_B1), ("b1", M.OFF_B1, M.OFF_W2), ("W2", M.OFF_W2, M.OFF_B2),
+                       ("b2", M.OFF_B2, M.OFF_W3), ("W3", M.OFF_W3, M.OFF_B3), ("b3", M.OFF_B3, M.OFF_W4),
+                       ("W4", M.OFF_W4, M.OFF_B4), ("b4", M.OFF_B4, M.NPARAM)):
+        rel = np.abs(g[a:b] - g_ref[a:b]).max() / np.abs(g_ref[a:b]).max()
+        print(f"grad {name}: max rel {rel:.2e}")
+        assert rel < 1e-4, name
+
+
+def test_exact_sampler_routine_matches_oracle(ctx):
+    t, cfg = ctx
+    rng = np.random.default_rng(5)
+    n = 20000
+    x = np.column_stack([rng.uniform(-4, 6, n), np.exp(rng.uniform(np.log(0.5), np.log(100), n)),
+                         rng.uniform(0.01, np.pi - 0.01, n)]).astype(np.float32)
+    got = t.mlp_exact(x[:, 0], x[:, 1], x[:, 2])
+    fj, fa = M.exact_lnf(x[:, 0].astype(np.float64), x[:, 1].astype(np.float64), x[:, 2].astype(np.float64),
+                         cfg["k2"], cfg["l0"], cfg["dl"])
+    assert np.max(np.abs(got[:, 0] - fj) / np.maximum(1, np.abs(fj))) < 1e-12
+    assert np.max(np.abs(got[:, 1] - fa) / np.maximum(1, np.abs(fa))) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    t, cfg = ctx
+    rep = t.mlp_train(n_samples=1 << 22, batch=8192, epochs=30, seed=1)
+    print(f"training: {rep}")
+    return rep
+
+
+def test_training_converges_and_is_accurate(ctx, trained):
+    t, cfg = ctx
+    assert trained["loss_last"] < trained["loss_first"] / 100
+    rng = np.random.default_rng(6)
+    n = 100000
+    # region that carries the 230 GHz frame's flux (emission-weighted 0.1 - 99.9 % quantiles)
+    u0, th, tb = rng.uniform(3.9, 4.9, n), np.exp(rng.uniform(np.log(10), np.log(42), n)), rng.uniform(0.25, 1.57, n)
+    got = t.mlp_infer(u0, th, tb)
+    ex = t.mlp_exact(u0, th, tb)
+    dj = np.abs(got[:, 0] - ex[:, 0])
+    print(f"surrogate |d ln f_j| in the emitting region: median {np.median(dj):.2e} p99 {np.percentile(dj, 99):.2e} "
+          f"max {dj.max():.2e}")
+    assert np.median(dj) < 1e-2
+
+
+def test_surrogate_in_tracer_and_gain(ctx, trained):
+    """What the trained network gains: mixed tracer with the baked surrogate vs the exact FP32
+    formula and vs the plain FP64 tracer on the 1024^2 frame."""
+    t, cfg = ctx
+    bake = t.mlp_bake(160, 64, 32, 2e-3)
+    print(f"bake: {bake}")
+    F = cfg["F"]
+    ref, st64 = t.render(F, bh.Tracer.opts(precision=bh.PREC_FP64))
+    mix, stx = t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED))
+    mlp, stm = t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_MLP))
+    Ib = ref[..., 0].astype(np.float64)
+    mask = Ib > 1e-3 * Ib.max()
+    for name, img, st in (("mixed+exact", mix, stx), ("mixed+mlp", mlp, stm)):
+        Ia = img[..., 0].astype(np.float64)
+        flux = abs(Ia.sum() - Ib.sum()) / Ib.sum()
+        med = np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask])
+        print(f"{name}: flux err {flux:.3e}, pixel median {med:.3e}, kernel {st.ms_kernel:.2f} ms "
+              f"(fp64 {st64.ms_kernel:.2f} ms, speed-up {st64.ms_kernel / st.ms_kernel:.2f}x)")
+    Ia = mlp[..., 0].astype(np.float64)
+    assert abs(Ia.sum() - Ib.sum()) / Ib.sum() < 1e-2
+    assert stm.ms_kernel < st64.ms_kernel
