@@ -156,6 +156,35 @@ int bh_radio_flux(bh_ctx* ctx, const bh_frame* frame, int n, double pix_sr, doub
 int bh_vacuum_fates(bh_ctx* ctx, const double* photons, int n, double tol, double r_escape, int max_steps,
                     int* fates);
 
+/* ---- Persistent batched tracing (new; the reference renders one frame per launch sequence,
+   render.cu:677-730). Frames of different size and scene (e.g. spin) are traced by one persistent
+   launch: resident workgroups pull 8 x 8 pixel tiles from per-frame work queues, retire finished
+   rays every `interval` steps and refill free lanes. The launch sequence (queue reset, persistent
+   kernel, statistics reduction) is captured once into a hipGraph and replayed. ---- */
+typedef struct bh_batch_frame {
+    bh_scene scene;     /* per-frame scene (tables and turbulence come from the context) */
+    bh_frame frame;     /* W, H, camera; one sample per pixel (centre or jittered sample 0) */
+} bh_batch_frame;
+
+typedef struct bh_batch_cfg {
+    int block;          /* threads per workgroup: 64, 128 or 256 (0 -> 256) */
+    int grid_per_cu;    /* resident workgroups per compute unit (0 -> 2) */
+    int interval;       /* integration steps between retire/refill (0 -> 8) */
+    int refill_min;     /* free lanes needed to refill (0 -> 16) */
+    int use_graph;      /* capture and replay the launch as a hipGraph */
+} bh_batch_cfg;
+
+/* Traces every frame of `frames`; px_out (nullable) receives the frames one after the other,
+   W*H*4 floats each. stats aggregates all frames. */
+int bh_trace_batch(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_trace_opts* opts,
+                   const bh_batch_cfg* cfg, float* px_out, bh_stats* stats);
+
+/* Times bh_trace_batch over a grid of configurations on the given workload and returns the fastest
+   (best) with its kernel time (best_ms). tried (nullable, 6 doubles per configuration: block,
+   grid_per_cu, interval, refill_min, ms, 0) receives up to max_tried rows; *n_tried their number. */
+int bh_autotune(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_trace_opts* opts, bh_batch_cfg* best,
+                double* best_ms, double* tried, int max_tried, int* n_tried);
+
 /* ---- Thermal-synchrotron surrogate (new: the reference has only the exact formula,
    blackholes/src/emission.cuh:159-170). A 3 -> 64 -> 64 -> 64 -> 2 SiLU MLP of
    (log10 nu/nu_c, Theta_e, theta_B) -> (ln f_j, ln f_a), f_j = j / (n_e e^2 nu_c / c),

@@ -375,12 +375,8 @@ void or_k2_table(int n, float* tab, float* l0f, float* dlf) {
 // Turbulent density factor: cell-centred trilinear interpolation of an n^3 grid over [-half, half]^3
 // at the Cartesian point (R cos phi, R sin phi, r u), R = r sin(theta). Replaces the texture
 // fetch of emission.cuh:112-123; the interpolation order below defines the value exactly.
-double or_turb_factor(const or_scene* S, double r, double u, double phi) {
+static double grid_trilinear(const or_scene* S, const float* G, const double p[3]) {
     const int n = S->turb_n;
-    double s = std::sqrt(std::fmax(1.0 - u * u, 1e-12));
-    double R = r * s, sp, cp;
-    M_SINCOS(phi, &sp, &cp);
-    double p[3] = {R * cp, R * sp, r * u};
     int i0[3];
     double f[3];
     for (int d = 0; d < 3; ++d) {
@@ -391,7 +387,6 @@ double or_turb_factor(const or_scene* S, double r, double u, double phi) {
         i0[d] = i;
         f[d] = g - (double)i;
     }
-    const float* G = S->turb_grid;
     size_t nn = (size_t)n;
     size_t b = ((size_t)i0[2] * nn + (size_t)i0[1]) * nn + (size_t)i0[0];
     size_t sy = nn, sz = nn * nn;
@@ -406,6 +401,25 @@ double or_turb_factor(const or_scene* S, double r, double u, double phi) {
     return c0 + (c1 - c0) * f[2];
 }
 
+// With time slices: linear in coordinate time t between the two slices around it, clamped to the
+// first and last slice.
+double or_turb_factor(const or_scene* S, double r, double u, double phi, double t) {
+    double s = std::sqrt(std::fmax(1.0 - u * u, 1e-12));
+    double R = r * s, sp, cp;
+    M_SINCOS(phi, &sp, &cp);
+    const double p[3] = {R * cp, R * sp, r * u};
+    if (S->turb_nt <= 1) return grid_trilinear(S, S->turb_grid, p);
+    double st = (t - S->turb_t0) * S->turb_inv_dt;
+    st = std::fmin(std::fmax(st, 0.0), (double)(S->turb_nt - 1));
+    int k = (int)st;
+    if (k > S->turb_nt - 2) k = S->turb_nt - 2;
+    const double f = st - (double)k;
+    const size_t slice = (size_t)S->turb_n * S->turb_n * S->turb_n;
+    const double A = grid_trilinear(S, S->turb_grid + (size_t)k * slice, p);
+    const double B = grid_trilinear(S, S->turb_grid + (size_t)(k + 1) * slice, p);
+    return A + (B - A) * f;
+}
+
 // blackholes/src/emission.cuh:20
 static double keplerian_omega(const or_scene* S, double R) { return 1.0 / (R * std::sqrt(R) + S->a); }
 
@@ -417,7 +431,6 @@ static double bessel_k2_inv(const or_scene* S, double theta) {
 // blackholes/src/emission.cuh:99-175
 void or_flow_step(const or_scene* S, double r, double u, double phi, double t, double lam, double Ecam, double h,
                   double* dI, double* dtau) {
-    (void)t;
     *dI = 0.0;
     *dtau = 0.0;
     const double a = S->a;
@@ -428,7 +441,7 @@ void or_flow_step(const or_scene* S, double r, double u, double phi, double t, d
     double vert = M_EXP(-0.5 * zh * zh);
     if (vert < 1e-7) return;
     double ne = S->n0 * M_EXP(-S->n_index * M_LOG(r)) * vert;
-    if (S->flow_turb_amp > 0.f && S->turb_grid && S->turb_n > 1) ne *= or_turb_factor(S, r, u, phi);
+    if (S->flow_turb_amp > 0.f && S->turb_grid && S->turb_n > 1) ne *= or_turb_factor(S, r, u, phi, t);
     ne *= 1.0 - M_EXP(-(r - S->rh) * (r - S->rh) / 0.25);
     if (ne < 1e-30) return;
     double Th = S->theta0 * M_EXP(-S->theta_index * M_LOG(r));

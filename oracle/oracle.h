@@ -28,10 +28,13 @@ typedef struct {
     double n0, n_index, theta_index, theta0, h_r, beta_p, r_flow_max, nu_obs, rg_cm, flow_kep;
     float flow_turb_amp;
     // Turbulence: density factor on a cell-centred n^3 Cartesian grid spanning [-half, half]^3,
-    // x fastest. n = 0 or grid == NULL disables it.
+    // x fastest. n = 0 or grid == NULL disables it. With turb_nt > 1 the grid holds turb_nt such
+    // slices at coordinate times turb_t0 + k / turb_inv_dt, blended linearly in time.
     const float* turb_grid;
     int turb_n;
     double turb_half, turb_inv_dx;
+    int turb_nt;
+    double turb_t0, turb_inv_dt;
     const float* k2_lntab;
     int k2_n;
     float k2_l0, k2_dl;
@@ -97,7 +100,7 @@ void or_sample_offsets(const or_frame* F, int px, int py, int s, int jitter, dou
 void or_k2_table(int n, float* tab, float* l0, float* dl);
 
 // emission.cuh / trace.cuh
-double or_turb_factor(const or_scene* S, double r, double u, double phi);
+double or_turb_factor(const or_scene* S, double r, double u, double phi, double t);
 void or_flow_step(const or_scene* S, double r, double u, double phi, double t, double lam, double Ecam,
                   double h, double* dI, double* dtau);
 void or_escape_angles(const double* y, const double* k, double* th, double* ph);

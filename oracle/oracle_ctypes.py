@@ -20,7 +20,7 @@ class OrScene(C.Structure):
         ("h_r", C.c_double), ("beta_p", C.c_double), ("r_flow_max", C.c_double), ("nu_obs", C.c_double),
         ("rg_cm", C.c_double), ("flow_kep", C.c_double), ("flow_turb_amp", C.c_float),
         ("turb_grid", C.POINTER(C.c_float)), ("turb_n", C.c_int), ("turb_half", C.c_double),
-        ("turb_inv_dx", C.c_double),
+        ("turb_inv_dx", C.c_double), ("turb_nt", C.c_int), ("turb_t0", C.c_double), ("turb_inv_dt", C.c_double),
         ("k2_lntab", C.POINTER(C.c_float)), ("k2_n", C.c_int), ("k2_l0", C.c_float), ("k2_dl", C.c_float),
         ("seed", C.c_uint32),
     ]
@@ -102,7 +102,7 @@ class Oracle:
         L.or_sample_offsets.argtypes = [C.POINTER(OrFrame), C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.or_k2_table.argtypes = [C.c_int, _F, _F, _F]
         L.or_turb_factor.restype = C.c_double
-        L.or_turb_factor.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.c_double]
+        L.or_turb_factor.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_flow_step.argtypes = [C.POINTER(OrScene)] + [C.c_double] * 7 + [_D, _D]
         L.or_escape_angles.argtypes = [_D, _D, _D, _D]
         L.or_trace_ray.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.c_double, C.c_double, C.POINTER(OrRay)]
@@ -211,6 +211,9 @@ class Oracle:
                                         rays.ctypes.data if rays is not None else None)
         return out_px, rays, used
 
+    def turb_factor(self, S, r, u, phi, t=0.0):
+        return self.lib.or_turb_factor(C.byref(S), r, u, phi, t)
+
     def flow_step(self, S, r, u, phi, t, lam, Ecam, h):
         dI = C.c_double()
         dt = C.c_double()
@@ -218,9 +221,10 @@ class Oracle:
         return dI.value, dt.value
 
 
-def make_scene(p, k2=None, grid=None, turb_half=60.0):
-    """OrScene from a dict of scene scalars (keys as in the struct). Keeps references to the arrays
-    on the returned object so they outlive the call."""
+def make_scene(p, k2=None, grid=None, turb_half=60.0, turb_t0=0.0, turb_dt=0.0):
+    """OrScene from a dict of scene scalars (keys as in the struct). grid: n^3, or nt x n^3 time
+    slices spaced turb_dt from turb_t0. Keeps references to the arrays on the returned object so
+    they outlive the call."""
     S = OrScene()
     for name, _ in OrScene._fields_:
         if name in p:
@@ -234,11 +238,14 @@ def make_scene(p, k2=None, grid=None, turb_half=60.0):
     if grid is not None:
         grid = np.ascontiguousarray(grid, dtype=np.float32)
         keep.append(grid)
-        n = grid.shape[0]
+        n = grid.shape[-1]
         S.turb_grid = grid.ctypes.data_as(_F)
         S.turb_n = n
         S.turb_half = turb_half
         S.turb_inv_dx = n / (2.0 * turb_half)
+        S.turb_nt = grid.shape[0] if grid.ndim == 4 else 1
+        S.turb_t0 = turb_t0
+        S.turb_inv_dt = 1.0 / turb_dt if turb_dt > 0 else 0.0
     S._keep = keep
     return S
 

@@ -23,6 +23,7 @@ EXPORTS = [
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
+    "bh_trace_batch", "bh_autotune",
 ]
 MLP_NPARAM = 8706
 
@@ -70,6 +71,18 @@ class Stats(C.Structure):
         ("step_limited", C.c_longlong), ("step_attempts", C.c_longlong), ("steps_accepted", C.c_longlong),
         ("flow_steps", C.c_longlong), ("fp64_steps", C.c_longlong), ("flux_sum", C.c_double),
     ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchFrame(C.Structure):
+    _fields_ = [("scene", Scene), ("frame", Frame)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("block", C.c_int), ("grid_per_cu", C.c_int), ("interval", C.c_int), ("refill_min", C.c_int),
+                ("use_graph", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -149,6 +162,12 @@ def load():
     L.bh_mlp_grad.argtypes = [C.c_void_p, _F, _F, _F, _F, _F, C.c_int, _F, _F]
     L.bh_mlp_bake.restype = C.c_int
     L.bh_mlp_bake.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(MlpBakeReport)]
+    L.bh_trace_batch.restype = C.c_int
+    L.bh_trace_batch.argtypes = [C.c_void_p, C.POINTER(BatchFrame), C.c_int, C.POINTER(TraceOpts),
+                                 C.POINTER(BatchCfg), _F, C.POINTER(Stats)]
+    L.bh_autotune.restype = C.c_int
+    L.bh_autotune.argtypes = [C.c_void_p, C.POINTER(BatchFrame), C.c_int, C.POINTER(TraceOpts), C.POINTER(BatchCfg),
+                              _D, _D, C.c_int, C.POINTER(C.c_int)]
     L.bh_math_eval.restype = C.c_int
     L.bh_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
     L.bh_destroy.argtypes = [C.c_void_p]
@@ -376,6 +395,46 @@ class Tracer:
         self._check(self.L.bh_vacuum_fates(self.ctx, ph.ctypes.data_as(_D), len(ph), tol, r_escape, max_steps,
                                            fates.ctypes.data_as(C.POINTER(C.c_int))), "bh_vacuum_fates")
         return fates
+
+    # --- persistent batches -------------------------------------------------------------------
+    @staticmethod
+    def _batch(frames):
+        arr = (BatchFrame * len(frames))()
+        for i, (S, F) in enumerate(frames):
+            arr[i].scene = S
+            arr[i].frame = F
+        return arr
+
+    def trace_batch(self, frames, opts=None, cfg=None, block=0, grid_per_cu=0, interval=0, refill_min=0,
+                    use_graph=1):
+        """frames: list of (Scene, Frame). Returns ([H x W x 4 images], Stats). cfg None and all launch
+        arguments 0 -> the last autotuned configuration, else the defaults."""
+        arr = self._batch(frames)
+        npix = sum(F.W * F.H for _, F in frames)
+        px = np.zeros((npix, 4), dtype=np.float32)
+        st = Stats()
+        o = opts or self.opts()
+        if cfg is None and (block or grid_per_cu or interval or refill_min or use_graph != 1):
+            cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph)
+        pc = C.byref(cfg) if cfg is not None else None
+        self._check(self.L.bh_trace_batch(self.ctx, arr, len(frames), C.byref(o), pc, px.ctypes.data_as(_F),
+                                          C.byref(st)), "bh_trace_batch")
+        out, off = [], 0
+        for _, F in frames:
+            out.append(px[off:off + F.W * F.H].reshape(F.H, F.W, 4))
+            off += F.W * F.H
+        return out, st
+
+    def autotune(self, frames, opts=None, max_tried=256):
+        arr = self._batch(frames)
+        best = BatchCfg()
+        ms = C.c_double()
+        tried = np.zeros((max_tried, 6), dtype=np.float64)
+        nt = C.c_int()
+        o = opts or self.opts()
+        self._check(self.L.bh_autotune(self.ctx, arr, len(frames), C.byref(o), C.byref(best), C.byref(ms),
+                                       tried.ctypes.data_as(_D), max_tried, C.byref(nt)), "bh_autotune")
+        return best, ms.value, tried[:min(nt.value, max_tried)]
 
     # --- surrogate ----------------------------------------------------------------------------
     def mlp_train(self, n_samples=0, batch=0, epochs=0, lr=0.0, lr_min=0.0, hot_weight=-1.0, seed=0):

@@ -22,6 +22,7 @@
 #include "trace_fp64.h"
 #include "trace_mixed.h"
 #include "ctx.h"
+#include "persist.h"
 
 using bhd::RayResult;
 using bhd::TraceParams;
@@ -153,6 +154,41 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
     }
 }
 
+template <int PREC, bool RADIO>
+__global__ __launch_bounds__(BLOCK) void k_persist(const bhp::BatchFrame* __restrict__ frames, int nframes,
+                                                   unsigned int* __restrict__ queue, int interval, int refill_min,
+                                                   BlockOut* __restrict__ blk) {
+    double flux;
+    unsigned long long c[NCNT];
+    bhp::persist_body<PREC, RADIO>(frames, nframes, queue, interval, refill_min, &flux, c);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __shared__ double s_flux[BLOCK / 64];
+    __shared__ unsigned long long s_cnt[BLOCK / 64][NCNT];
+    flux = wave_sum(flux);
+    for (int i = 0; i < NCNT; ++i) c[i] = wave_sum(c[i]);
+    if (lane == 0) {
+        s_flux[wave] = flux;
+        for (int i = 0; i < NCNT; ++i) s_cnt[wave][i] = c[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BlockOut b;
+        b.flux = 0.0;
+        for (int i = 0; i < NCNT; ++i) b.cnt[i] = 0;
+        for (int w = 0; w < nw; ++w) {
+            b.flux += s_flux[w];
+            for (int i = 0; i < NCNT; ++i) b.cnt[i] += s_cnt[w][i];
+        }
+        blk[blockIdx.x] = b;
+    }
+}
+
+// graph-safe reset of the work queues (a kernel, not a memset node)
+__global__ void k_zero_u32(unsigned int* __restrict__ p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(64) void k_vacuum_fates(const TraceParams P, const double* __restrict__ photons, int n,
                                                      double tol, double r_escape, int max_steps,
                                                      int* __restrict__ fates) {
@@ -223,10 +259,9 @@ int ensure(bh_ctx* c, T** p, size_t* cap, size_t n) {
     return BH_OK;
 }
 
-TraceParams make_params(const bh_ctx* c, const bh_frame* f) {
+TraceParams make_params_scene(const bh_ctx* c, const bh_scene& S, const bh_frame* f) {
     TraceParams P;
     std::memset(&P, 0, sizeof P);
-    const bh_scene& S = c->S;
     P.a = S.a;
     P.a2 = S.a * S.a;
     P.rh = S.rh;
@@ -281,6 +316,8 @@ TraceParams make_params(const bh_ctx* c, const bh_frame* f) {
     return P;
 }
 
+TraceParams make_params(const bh_ctx* c, const bh_frame* f) { return make_params_scene(c, c->S, f); }
+
 int check_frame(bh_ctx* c, const bh_frame* f) {
     if (!f) return fail(c, BH_ERR_ARG, "frame is NULL");
     if (f->W <= 0 || f->H <= 0 || f->W > 65536 || f->H > 65536) return fail(c, BH_ERR_ARG, "bad frame size");
@@ -334,6 +371,16 @@ bhm32::MixedParams make_mixed(const bh_scene& S, const bh_trace_opts& o) {
     return M;
 }
 
+int check_opts(bh_ctx* c, const bh_trace_opts& o) {
+    if (o.precision != BH_PREC_FP64 && o.precision != BH_PREC_MIXED) return fail(c, BH_ERR_ARG, "bad precision");
+    if (o.emissivity != BH_EMIS_EXACT && o.emissivity != BH_EMIS_MLP) return fail(c, BH_ERR_ARG, "bad emissivity");
+    if (o.emissivity == BH_EMIS_MLP && o.precision != BH_PREC_MIXED)
+        return fail(c, BH_ERR_ARG, "the MLP emissivity runs in the mixed-precision tracer");
+    if (o.emissivity == BH_EMIS_MLP && !c->stab_ready)
+        return fail(c, BH_ERR_UNSUPPORTED, "no baked surrogate: call bh_mlp_train (or bh_mlp_set) and bh_mlp_bake first");
+    return BH_OK;
+}
+
 // Launches the trace of a w x h tile at (x0, y0) and the block reduction; fills stats counters.
 int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0, int y0, int w, int h, bool rays,
                  bh_stats* st) {
@@ -342,12 +389,8 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     if (w <= 0 || h <= 0) return fail(c, BH_ERR_ARG, "empty tile");
     bh_trace_opts o{};
     if (opts) o = *opts;
-    if (o.precision != BH_PREC_FP64 && o.precision != BH_PREC_MIXED) return fail(c, BH_ERR_ARG, "bad precision");
-    if (o.emissivity != BH_EMIS_EXACT && o.emissivity != BH_EMIS_MLP) return fail(c, BH_ERR_ARG, "bad emissivity");
-    if (o.emissivity == BH_EMIS_MLP && o.precision != BH_PREC_MIXED)
-        return fail(c, BH_ERR_ARG, "the MLP emissivity runs in the mixed-precision tracer");
-    if (o.emissivity == BH_EMIS_MLP && !c->stab_ready)
-        return fail(c, BH_ERR_UNSUPPORTED, "no baked surrogate: call bh_mlp_train (or bh_mlp_set) and bh_mlp_bake first");
+    rc = check_opts(c, o);
+    if (rc) return rc;
     bhm32::MixedParams M = make_mixed(c->S, o);
     if (o.emissivity == BH_EMIS_MLP) {
         M.use_surrogate = 1;
@@ -384,6 +427,149 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
         HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         st->ms_kernel = ms;
         st->samples = (long long)w * h * P.spp;
+        st->step_attempts = (long long)tot.cnt[0];
+        st->steps_accepted = (long long)tot.cnt[1];
+        st->flow_steps = (long long)tot.cnt[2];
+        st->step_limited = (long long)tot.cnt[3];
+        st->fp64_steps = (long long)tot.cnt[4];
+        st->flux_sum = tot.flux;
+    }
+    return BH_OK;
+}
+
+using PersistFn = void (*)(const bhp::BatchFrame*, int, unsigned int*, int, int, BlockOut*);
+
+PersistFn pick_persist(int prec, bool radio) {
+    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true> : k_persist<BH_PREC_FP64, false>;
+    return radio ? k_persist<BH_PREC_MIXED, true> : k_persist<BH_PREC_MIXED, false>;
+}
+
+constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
+
+bh_batch_cfg batch_defaults() {
+    bh_batch_cfg d{};
+    d.block = 256;
+    d.grid_per_cu = 2;
+    d.interval = 8;
+    d.refill_min = 16;
+    d.use_graph = 1;
+    return d;
+}
+
+// Traces a batch with one persistent launch: [queue reset, k_persist, k_reduce], captured into a
+// hipGraph (rebuilt when a buffer, the kernel or the launch shape changes) or launched directly.
+int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_opts* opts, const bh_batch_cfg* cfg_in,
+                 long long* npix_out, bh_stats* st) {
+    if (!frames || n <= 0) return fail(c, BH_ERR_ARG, "empty batch");
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    bh_batch_cfg cfg = cfg_in ? *cfg_in : (c->tuned_ok ? c->tuned : batch_defaults());
+    const bh_batch_cfg d = batch_defaults();
+    if (cfg.block == 0) cfg.block = d.block;
+    if (cfg.grid_per_cu == 0) cfg.grid_per_cu = d.grid_per_cu;
+    if (cfg.interval == 0) cfg.interval = d.interval;
+    if (cfg.refill_min == 0) cfg.refill_min = d.refill_min;
+    if (cfg.block != 64 && cfg.block != 128 && cfg.block != 256) return fail(c, BH_ERR_ARG, "block must be 64, 128 or 256");
+    if (cfg.grid_per_cu < 1 || cfg.grid_per_cu > 32) return fail(c, BH_ERR_ARG, "grid_per_cu must be in [1, 32]");
+    if (cfg.interval < 1 || cfg.interval > 4096) return fail(c, BH_ERR_ARG, "interval must be in [1, 4096]");
+    if (cfg.refill_min < 1 || cfg.refill_min > 64) return fail(c, BH_ERR_ARG, "refill_min must be in [1, 64]");
+    const int mode = frames[0].scene.mode;
+    long long total = 0;
+    std::vector<bhp::BatchFrame> hb((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const bh_frame* f = &frames[i].frame;
+        const bh_scene& S = frames[i].scene;
+        if (f->W <= 0 || f->H <= 0 || (long long)f->W * f->H > BATCH_MAX_PIXELS)
+            return fail(c, BH_ERR_ARG, "bad frame size in batch (at most 2^30 pixels per frame)");
+        if (!(f->pix > 0.0)) return fail(c, BH_ERR_ARG, "frame pixel scale must be > 0");
+        if (S.mode != mode) return fail(c, BH_ERR_ARG, "all frames of a batch must share the render mode");
+        if (S.mode == BH_MODE_RADIO && S.flow_on && !c->d_k2) return fail(c, BH_ERR_ARG, "radio mode needs the K2 table");
+        bhp::BatchFrame& B = hb[(size_t)i];
+        std::memset(&B, 0, sizeof B);
+        B.P = make_params_scene(c, S, f);
+        B.P.w = f->W;
+        B.P.h = f->H;
+        B.P.spp = 1;
+        B.P.jitter = o.jitter;
+        B.M = make_mixed(S, o);
+        if (o.emissivity == BH_EMIS_MLP) {
+            B.M.use_surrogate = 1;
+            B.M.S = c->stab;
+        }
+        B.tiles_x = (f->W + 7) / 8;
+        B.nentries = (unsigned int)((long long)B.tiles_x * ((f->H + 7) / 8) * 64);
+        total += (long long)f->W * f->H;
+    }
+    rc = ensure(c, &c->d_bout, &c->bout_cap, (size_t)total);
+    if (rc) return rc;
+    long long off = 0;
+    for (int i = 0; i < n; ++i) {
+        hb[(size_t)i].out = c->d_bout + off;
+        off += (long long)frames[i].frame.W * frames[i].frame.H;
+    }
+    const int grid = c->n_cu * cfg.grid_per_cu;
+    rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof(bhp::BatchFrame) * (size_t)n);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_queue, &c->queue_cap, (size_t)n);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid);
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_bframes, hb.data(), sizeof(bhp::BatchFrame) * (size_t)n, hipMemcpyHostToDevice, c->st));
+    PersistFn k = pick_persist(o.precision, mode == BH_MODE_RADIO);
+    const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
+    auto enqueue = [&]() -> hipError_t {
+        hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), 0, c->st, dframes, n, c->d_queue, cfg.interval,
+                           cfg.refill_min, c->d_blk);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
+        return hipGetLastError();
+    };
+    HIPCK(c, hipEventRecord(c->ev0, c->st));
+    if (cfg.use_graph) {
+        // everything the captured launches depend on, except the frame descriptors (copied above)
+        unsigned long long key = 1469598103934665603ull;
+        const unsigned long long parts[] = {(unsigned long long)(uintptr_t)k, (unsigned long long)(uintptr_t)dframes,
+                                            (unsigned long long)(uintptr_t)c->d_queue,
+                                            (unsigned long long)(uintptr_t)c->d_blk,
+                                            (unsigned long long)(uintptr_t)c->d_tot, (unsigned long long)n,
+                                            (unsigned long long)grid, (unsigned long long)cfg.block,
+                                            (unsigned long long)cfg.interval, (unsigned long long)cfg.refill_min};
+        for (unsigned long long v : parts) key = (key ^ v) * 1099511628211ull;
+        if (!c->graph || c->graph_key != key) {
+            if (c->graph) {
+                HIPCK(c, hipGraphExecDestroy(c->graph));
+                c->graph = nullptr;
+            }
+            if (c->graph_def) {
+                HIPCK(c, hipGraphDestroy(c->graph_def));
+                c->graph_def = nullptr;
+            }
+            HIPCK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            hipError_t le = enqueue();
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(c->st, &g);
+            HIPCK(c, le);
+            HIPCK(c, ce);
+            c->graph_def = g;
+            HIPCK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+            c->graph_key = key;
+        }
+        HIPCK(c, hipGraphLaunch(c->graph, c->st));
+    } else {
+        HIPCK(c, enqueue());
+    }
+    HIPCK(c, hipEventRecord(c->ev1, c->st));
+    BlockOut tot;
+    HIPCK(c, hipMemcpyAsync(&tot, c->d_tot, sizeof tot, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    if (npix_out) *npix_out = total;
+    if (st) {
+        float ms = 0.f;
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->ms_kernel = ms;
+        st->samples = total;
         st->step_attempts = (long long)tot.cnt[0];
         st->steps_accepted = (long long)tot.cnt[1];
         st->flow_steps = (long long)tot.cnt[2];
@@ -444,6 +630,10 @@ int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const 
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc((void**)&c->d_tot, sizeof(BlockOut)) != hipSuccess) {
         c->err = "stream/event/alloc creation failed";
+        return bail(BH_ERR_HIP);
+    }
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->n_cu <= 0) {
+        c->err = "cannot query the compute-unit count";
         return bail(BH_ERR_HIP);
     }
     if (tables) {
@@ -556,6 +746,82 @@ int bh_radio_flux(bh_ctx* c, const bh_frame* f, int n, double pix_sr, double* jy
     return BH_OK;
 }
 
+int bh_trace_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_opts* opts,
+                   const bh_batch_cfg* cfg, float* px_out, bh_stats* st) {
+    if (!c) return BH_ERR_ARG;
+    double t0 = now_ms();
+    HIPCK(c, hipSetDevice(c->device));
+    bh_stats local{};
+    long long npix = 0;
+    int rc = launch_batch(c, frames, n, opts, cfg, &npix, &local);
+    if (rc) return rc;
+    if (px_out) {
+        HIPCK(c, hipMemcpyAsync(px_out, c->d_bout, sizeof(float4) * (size_t)npix, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+    }
+    local.ms_trace = now_ms() - t0;
+    if (st) *st = local;
+    return BH_OK;
+}
+
+int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_opts* opts, bh_batch_cfg* best,
+                double* best_ms, double* tried, int max_tried, int* n_tried) {
+    if (!c || !best) return BH_ERR_ARG;
+    HIPCK(c, hipSetDevice(c->device));
+    // coordinate descent from the defaults: one parameter at a time over its candidate list, two passes
+    bh_batch_cfg cur = batch_defaults();
+    int count = 0;
+    double cur_ms = 0.0;
+    auto time_cfg = [&](const bh_batch_cfg& g, double* ms) -> int {
+        bh_stats s{};
+        int rc = launch_batch(c, frames, n, opts, &g, nullptr, &s);   // first run: warm caches and graph
+        if (rc) return rc;
+        rc = launch_batch(c, frames, n, opts, &g, nullptr, &s);
+        if (rc) return rc;
+        *ms = s.ms_kernel;
+        if (tried && count < max_tried) {
+            double* row = tried + 6 * (size_t)count;
+            row[0] = g.block;
+            row[1] = g.grid_per_cu;
+            row[2] = g.interval;
+            row[3] = g.refill_min;
+            row[4] = s.ms_kernel;
+            row[5] = 0.0;
+        }
+        ++count;
+        return BH_OK;
+    };
+    int rc = time_cfg(cur, &cur_ms);
+    if (rc) return rc;
+    const int blocks[] = {64, 128, 256}, grids[] = {1, 2, 3, 4, 8}, intervals[] = {2, 4, 8, 16, 32, 64},
+              refills[] = {1, 8, 16, 32, 48};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int p = 0; p < 4; ++p) {
+            const int* vals = p == 0 ? blocks : p == 1 ? grids : p == 2 ? intervals : refills;
+            const int nv = p == 0 ? 3 : p == 1 ? 5 : p == 2 ? 6 : 5;
+            for (int i = 0; i < nv; ++i) {
+                bh_batch_cfg g = cur;
+                int* field = p == 0 ? &g.block : p == 1 ? &g.grid_per_cu : p == 2 ? &g.interval : &g.refill_min;
+                if (*field == vals[i]) continue;
+                *field = vals[i];
+                double ms;
+                rc = time_cfg(g, &ms);
+                if (rc) return rc;
+                if (ms < cur_ms) {
+                    cur = g;
+                    cur_ms = ms;
+                }
+            }
+        }
+    }
+    *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    if (n_tried) *n_tried = count;
+    c->tuned = cur;
+    c->tuned_ok = true;
+    return BH_OK;
+}
+
 int bh_vacuum_fates(bh_ctx* c, const double* photons, int n, double tol, double r_escape, int max_steps, int* fates) {
     if (!c || !photons || !fates || n < 0) return BH_ERR_ARG;
     if (n == 0) return BH_OK;
@@ -610,8 +876,10 @@ void bh_destroy(bh_ctx* c) {
     // best-effort teardown: errors here have nowhere to go
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* bufs[] = {c->d_k2,  c->d_turb, c->d_px,     c->d_rays,     c->d_blk,     c->d_tot,
-                    c->d_ph,  c->d_fates, c->d_mlp, c->d_adam_m, c->d_adam_v, c->d_stab, c->d_sbad};
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph_def) (void)hipGraphDestroy(c->graph_def);
+    void* bufs[] = {c->d_k2,  c->d_turb,  c->d_px,  c->d_rays,   c->d_blk,    c->d_tot,  c->d_ph,    c->d_fates,
+                    c->d_mlp, c->d_adam_m, c->d_adam_v, c->d_stab, c->d_sbad, c->d_bframes, c->d_queue, c->d_bout};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -47,5 +47,18 @@ struct bh_ctx {
     unsigned char* d_sbad = nullptr;
     bhmlp::SurrogateTable stab{};
     bool stab_ready = false;
+    // persistent batches: frame descriptors, per-frame queues, output, cached graph
+    unsigned char* d_bframes = nullptr;   // bhp::BatchFrame[]
+    size_t bframes_cap = 0;               // bytes
+    unsigned int* d_queue = nullptr;
+    size_t queue_cap = 0;
+    float4* d_bout = nullptr;
+    size_t bout_cap = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    unsigned long long graph_key = 0;
+    int n_cu = 0;
+    bh_batch_cfg tuned{};   // last bh_autotune result (used when bh_trace_batch gets no cfg)
+    bool tuned_ok = false;
     std::string err;
 };

@@ -1,0 +1,154 @@
+/* persist.h - persistent multi-frame tracer (device code, included by bhrt.hip).
+
+   Workgroups stay resident and pull rays from per-frame work queues (one atomic counter per frame;
+   queue entries are the rays of 8 x 8 pixel tiles in tile order, so the rays a wave takes at once are
+   image neighbours). Every `interval` integration steps a wave retires finished lanes (one ballot),
+   writes their pixels, and refills the free lanes with new rays when at least `refill_min` lanes are
+   free: lane 0 claims a contiguous range with one atomic add and each free lane takes the entry at
+   its rank among the free lanes. A wave refills only from its current frame and moves to the next
+   frame once its lanes are drained, so the frame parameters stay wave-uniform (scalar loads).
+   Every wave exits when all frames are exhausted and its lanes are idle; each ray runs at most
+   max_steps steps, so the grid always drains. */
+#pragma once
+#include "trace_fp64.h"
+#include "trace_mixed.h"
+
+namespace bhp {
+
+using bhd::RayResult;
+using bhd::TraceParams;
+
+struct BatchFrame {
+    TraceParams P;         // scene + frame; P.w/P.h = frame size, P.spp = 1
+    bhm32::MixedParams M;  // the frame's mixed-precision policy and emission constants
+    unsigned int nentries; // 64 * (8x8 tiles)
+    int tiles_x;
+    float4* out;           // W*H pixels
+};
+
+struct BatchCounters {
+    double flux;
+    unsigned long long cnt[5];
+};
+
+__device__ __forceinline__ bool entry_pixel(const BatchFrame& B, unsigned int e, int* px, int* py) {
+    const unsigned int tile = e >> 6, within = e & 63u;
+    *px = (int)(tile % (unsigned)B.tiles_x) * 8 + (int)(within & 7u);
+    *py = (int)(tile / (unsigned)B.tiles_x) * 8 + (int)(within >> 3);
+    return *px < B.P.w && *py < B.P.h;
+}
+
+template <int PREC, bool RADIO>
+struct Tracer;
+
+template <bool RADIO>
+struct Tracer<BH_PREC_FP64, RADIO> {
+    using State = bhd::RayState;
+    static __device__ __forceinline__ void init(const TraceParams& P, const bhm32::MixedParams&, double x, double y,
+                                                State& R) {
+        bhd::ray_init<RADIO>(P, x, y, R);
+    }
+    static __device__ __forceinline__ void step(const TraceParams& P, const bhm32::MixedParams&, State& R) {
+        bhd::ray_step<RADIO>(P, R);
+    }
+    static __device__ __forceinline__ int result(const State& R, RayResult& out) {
+        bhd::ray_result(R, out);
+        return out.steps;
+    }
+};
+
+template <bool RADIO>
+struct Tracer<BH_PREC_MIXED, RADIO> {
+    using State = bhm32::RayStateM;
+    static __device__ __forceinline__ void init(const TraceParams& P, const bhm32::MixedParams& M, double x, double y,
+                                                State& R) {
+        bhm32::ray_init_m<RADIO>(P, M, x, y, R);
+    }
+    static __device__ __forceinline__ void step(const TraceParams& P, const bhm32::MixedParams& M, State& R) {
+        bhm32::ray_step_m<RADIO>(P, M, R);
+    }
+    static __device__ __forceinline__ int result(const State& R, RayResult& out) {
+        return bhm32::ray_result_m(R, out);
+    }
+};
+
+template <int PREC, bool RADIO>
+__device__ __forceinline__ void persist_body(const BatchFrame* __restrict__ frames, int nframes,
+                                             unsigned int* __restrict__ queue, int interval, int refill_min,
+                                             double* flux_out,
+                                             unsigned long long* cnt_out) {
+    using T = Tracer<PREC, RADIO>;
+    const int lane = threadIdx.x & 63;
+    int cur = 0;   // the wave's frame (wave-uniform)
+    bool active = false;
+    int px = 0, py = 0;
+    typename T::State R;
+    double flux = 0.0;
+    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    for (;;) {
+        // retire-and-refill
+        const unsigned long long freem = __ballot(!active);
+        const int nfree = __popcll(freem);
+        if (nfree >= refill_min || nfree == 64) {
+            while (cur < nframes) {
+                const unsigned int n = frames[cur].nentries;
+                unsigned int base = 0;
+                if (lane == 0) {
+                    // read before adding, so an exhausted counter grows by at most one claim per wave
+                    base = __hip_atomic_load(&queue[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (base < n) base = atomicAdd(&queue[cur], (unsigned int)nfree);
+                }
+                base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+                if (base < n) {
+                    if (!active) {
+                        const unsigned int rank = (unsigned int)__popcll(freem & ((1ull << lane) - 1ull));
+                        const unsigned int e = base + rank;
+                        if (e < n && entry_pixel(frames[cur], e, &px, &py)) {
+                            const TraceParams& P = frames[cur].P;
+                            double ox, oy;
+                            bhd::sample_offsets(P, px, py, 0, &ox, &oy);
+                            T::init(P, frames[cur].M, ox, oy, R);
+                            active = true;
+                        }
+                    }
+                    break;
+                }
+                if (nfree == 64) {
+                    ++cur;   // this frame's queue is empty and the wave is idle: next frame
+                    continue;
+                }
+                break;   // tail of the frame: keep stepping the live lanes
+            }
+        }
+        if (__ballot(active) == 0ull && cur >= nframes) break;
+        const int cf = __builtin_amdgcn_readfirstlane(cur < nframes ? cur : nframes - 1);
+        const TraceParams& P = frames[cf].P;
+        const bhm32::MixedParams& M = frames[cf].M;
+        for (int k = 0; k < interval; ++k) {
+            if (active) {
+                if (!R.done) T::step(P, M, R);
+                if (R.done) {
+                    RayResult r;
+                    const int f64 = T::result(R, r);
+                    frames[cf].out[(size_t)py * P.w + px] =
+                        make_float4((float)r.I[0], (float)r.I[1], (float)r.I[2], (float)r.I[3]);
+                    flux += r.I[0];
+                    c0 += (unsigned long long)r.steps;
+                    c1 += (unsigned long long)r.accepted;
+                    c2 += (unsigned long long)r.flow_steps;
+                    c3 += r.fate == 0 ? 1ull : 0ull;
+                    c4 += (unsigned long long)f64;
+                    active = false;
+                }
+            }
+        }
+    }
+    *flux_out = flux;
+    cnt_out[0] = c0;
+    cnt_out[1] = c1;
+    cnt_out[2] = c2;
+    cnt_out[3] = c3;
+    cnt_out[4] = c4;
+}
+
+}  // namespace bhp

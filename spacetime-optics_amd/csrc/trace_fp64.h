@@ -361,103 +361,136 @@ __device__ __forceinline__ double initial_step(const State& s) {
     return 0.02 * fmin(s.y0 / (fabs(s.y1) + 1e-12), 1.0 / (fabs(s.y3) + 1e-3));
 }
 
+// Per-ray integration state of trace_ray (trace.cuh:62-80), kept in registers between steps.
+struct RayState {
+    RayConst c;
+    State s;
+    double h, acc0, acc1, acc2, acc3, trans, psi, ppx, ppy, ppz;
+    int ncross, accepted, flow_steps, it, fate;
+    bool done;
+};
+
+// blackholes/src/trace.cuh:62-80: initial state of the ray through image-plane offset (x, yv)
+template <bool RADIO>
+__device__ __forceinline__ void ray_init(const TraceParams& P, double x, double yv, RayState& R) {
+    camera_photon(P, x, yv, R.c, R.s);
+    {
+        Rhs k = rhs(P, R.c, R.s.y0, R.s.y2);
+        R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
+    }
+    R.h = initial_step(R.s);
+    R.acc0 = R.acc1 = R.acc2 = R.acc3 = 0.0;
+    R.trans = 1.0;
+    R.psi = 0.0;
+    R.ncross = R.accepted = R.flow_steps = R.it = R.fate = 0;
+    R.ppx = R.ppy = R.ppz = 0.0;
+    if (RADIO) {
+        double sp = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0)), sv, cv;
+        bhm::sincos(R.s.y4, &sv, &cv);
+        R.ppx = sp * cv;
+        R.ppy = sp * sv;
+        R.ppz = R.s.y2;
+    }
+    R.done = P.max_steps <= 0;
+}
+
+// One iteration of trace.cuh:81-163 (one DP54 attempt); sets R.done when the ray stops.
+template <bool RADIO>
+__device__ __forceinline__ void ray_step(const TraceParams& P, RayState& R) {
+    const bool flow = RADIO && P.flow_on;
+    State n;
+    double hc = R.h;
+    if (flow && R.s.y0 < P.r_flow_max) {
+        double sn = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0));
+        double cap = P.step_frac * fmin(R.s.y0 / (fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) / (fabs(R.s.y3) + 1e-30));
+        hc = fmin(hc, cap);
+    }
+    double err = dp54_step(P, R.c, R.s, hc, n);
+    ++R.it;
+    if (!(err <= 1.0)) {
+        R.h = hc * fmax(0.2, 0.9 * bhm::pow_m02(fmax(err, 1e-30)));
+        if (!(err == err)) R.h = hc * 0.1;
+        if (R.it >= P.max_steps) R.done = true;
+        return;
+    }
+    ++R.accepted;
+    if (RADIO) {
+        double sn = sqrt(fmax(1.0 - n.y2 * n.y2, 0.0)), sv, cv;
+        bhm::sincos(n.y4, &sv, &cv);
+        double pnx = sn * cv, pny = sn * sv, pnz = n.y2;
+        double cx = R.ppy * pnz - R.ppz * pny, cy = R.ppz * pnx - R.ppx * pnz, cz = R.ppx * pny - R.ppy * pnx;
+        double cl = sqrt(cx * cx + cy * cy + cz * cz);
+        double dt = R.ppx * pnx + R.ppy * pny + R.ppz * pnz;
+        R.psi += bhm::atan2(cl, dt);
+        R.ppx = pnx;
+        R.ppy = pny;
+        R.ppz = pnz;
+    }
+    if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
+    if (flow && n.y0 < P.r_flow_max) {
+        double dI, dtau;
+        ++R.flow_steps;
+        flow_step(P, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, &dI, &dtau);
+        double cI = R.trans * dI;
+        int ord = (int)(R.psi * (1.0 / PI));
+        R.acc0 += cI;
+        if (ord == 1) R.acc1 += cI;
+        else if (ord == 2) R.acc2 += cI;
+        else if (ord >= 3) R.acc3 += cI;
+        R.trans *= bhm::exp(-dtau);
+        if (R.trans < 1e-5) {
+            R.fate = -1;
+            R.done = true;
+            return;
+        }
+    }
+    R.s = n;
+    project_radial(P, R.c, R.s);
+    R.h = hc * fmin(5.0, fmax(0.2, 0.9 * bhm::pow_m02(fmax(err, 1e-30))));
+    if (R.s.y0 < P.r_stop) {
+        R.fate = -1;
+        R.done = true;
+        return;
+    }
+    if (R.s.y0 > P.r_escape && R.s.y1 > 0.0) {
+        R.fate = 1;   // escape angles are taken from the final state in ray_result
+        R.done = true;
+        return;
+    }
+    if (R.it >= P.max_steps) R.done = true;
+}
+
+__device__ __forceinline__ void ray_result(const RayState& R, RayResult& out) {
+    out.y[0] = R.s.y0; out.y[1] = R.s.y1; out.y[2] = R.s.y2; out.y[3] = R.s.y3; out.y[4] = R.s.y4;
+    out.y[5] = R.s.y5;
+    out.th_inf = 0.0;
+    out.ph_inf = 0.0;
+    if (R.fate == 1) {
+        // blackholes/src/trace.cuh:49-55 (k1[4] is the phi derivative of the final state)
+        double sq = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 1e-30));
+        double tq = bhm::acos(fmax(-1.0, fmin(1.0, R.s.y2)));
+        double dthdr = (-R.s.y3 / sq) / R.s.y1, dphdr = R.s.k4 / R.s.y1;
+        out.th_inf = tq + R.s.y0 * dthdr;
+        out.ph_inf = R.s.y4 + R.s.y0 * dphdr;
+    }
+    out.I[0] = R.acc0; out.I[1] = R.acc1; out.I[2] = R.acc2; out.I[3] = R.acc3;
+    out.trans = R.trans;
+    out.psi = R.psi;
+    out.fate = R.fate;
+    out.ncross = R.ncross;
+    out.order = (int)(R.psi * (1.0 / PI));
+    out.steps = R.it;
+    out.accepted = R.accepted;
+    out.flow_steps = R.flow_steps;
+}
+
 // blackholes/src/trace.cuh:62-165 (disk and sky branches out of scope), FP64 intensity.
 template <bool RADIO>
 __device__ __forceinline__ void trace_ray_fp64(const TraceParams& P, double x, double yv, RayResult& out) {
-    RayConst c;
-    State s, n;
-    camera_photon(P, x, yv, c, s);
-    {
-        Rhs k = rhs(P, c, s.y0, s.y2);
-        s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
-    }
-    double h = initial_step(s);
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-    double trans = 1.0, psi = 0.0;
-    int ncross = 0, accepted = 0, flow_steps = 0, fate = 0, it = 0;
-    double th_inf = 0.0, ph_inf = 0.0;
-    double ppx = 0.0, ppy = 0.0, ppz = 0.0;
-    const bool flow = RADIO && P.flow_on;
-    if (RADIO) {
-        double sp = sqrt(fmax(1.0 - s.y2 * s.y2, 0.0)), sv, cv;
-        bhm::sincos(s.y4, &sv, &cv);
-        ppx = sp * cv;
-        ppy = sp * sv;
-        ppz = s.y2;
-    }
-    for (it = 0; it < P.max_steps; ++it) {
-        double hc = h;
-        if (flow && s.y0 < P.r_flow_max) {
-            double sn = sqrt(fmax(1.0 - s.y2 * s.y2, 0.0));
-            double cap = P.step_frac * fmin(s.y0 / (fabs(s.y1) + 1e-30), fmax(sn, 0.05) / (fabs(s.y3) + 1e-30));
-            hc = fmin(hc, cap);
-        }
-        double err = dp54_step(P, c, s, hc, n);
-        if (!(err <= 1.0)) {
-            h = hc * fmax(0.2, 0.9 * bhm::pow_m02(fmax(err, 1e-30)));
-            if (!(err == err)) h = hc * 0.1;
-            continue;
-        }
-        ++accepted;
-        if (RADIO) {
-            double sn = sqrt(fmax(1.0 - n.y2 * n.y2, 0.0)), sv, cv;
-            bhm::sincos(n.y4, &sv, &cv);
-            double pnx = sn * cv, pny = sn * sv, pnz = n.y2;
-            double cx = ppy * pnz - ppz * pny, cy = ppz * pnx - ppx * pnz, cz = ppx * pny - ppy * pnx;
-            double cl = sqrt(cx * cx + cy * cy + cz * cz);
-            double dt = ppx * pnx + ppy * pny + ppz * pnz;
-            psi += bhm::atan2(cl, dt);
-            ppx = pnx;
-            ppy = pny;
-            ppz = pnz;
-        }
-        if ((s.y2 > 0.0) != (n.y2 > 0.0)) ++ncross;
-        if (flow && n.y0 < P.r_flow_max) {
-            double dI, dtau;
-            ++flow_steps;
-            flow_step(P, n.y0, n.y2, n.y4, n.y5, c.lam, c.E, hc, &dI, &dtau);
-            double cI = trans * dI;
-            int ord = (int)(psi * (1.0 / PI));
-            acc0 += cI;
-            if (ord == 1) acc1 += cI;
-            else if (ord == 2) acc2 += cI;
-            else if (ord >= 3) acc3 += cI;
-            trans *= bhm::exp(-dtau);
-            if (trans < 1e-5) {
-                fate = -1;
-                break;
-            }
-        }
-        s = n;
-        project_radial(P, c, s);
-        h = hc * fmin(5.0, fmax(0.2, 0.9 * bhm::pow_m02(fmax(err, 1e-30))));
-        if (s.y0 < P.r_stop) {
-            fate = -1;
-            break;
-        }
-        if (s.y0 > P.r_escape && s.y1 > 0.0) {
-            fate = 1;
-            // blackholes/src/trace.cuh:49-55 (k1[4] is the phi derivative of the final state)
-            double sq = sqrt(fmax(1.0 - s.y2 * s.y2, 1e-30));
-            double tq = bhm::acos(fmax(-1.0, fmin(1.0, s.y2)));
-            double dthdr = (-s.y3 / sq) / s.y1, dphdr = s.k4 / s.y1;
-            th_inf = tq + s.y0 * dthdr;
-            ph_inf = s.y4 + s.y0 * dphdr;
-            break;
-        }
-    }
-    out.y[0] = s.y0; out.y[1] = s.y1; out.y[2] = s.y2; out.y[3] = s.y3; out.y[4] = s.y4; out.y[5] = s.y5;
-    out.th_inf = th_inf;
-    out.ph_inf = ph_inf;
-    out.I[0] = acc0; out.I[1] = acc1; out.I[2] = acc2; out.I[3] = acc3;
-    out.trans = trans;
-    out.psi = psi;
-    out.fate = fate;
-    out.ncross = ncross;
-    out.order = (int)(psi * (1.0 / PI));
-    out.steps = it < P.max_steps ? it + 1 : P.max_steps;
-    out.accepted = accepted;
-    out.flow_steps = flow_steps;
+    RayState R;
+    ray_init<RADIO>(P, x, yv, R);
+    while (!R.done) ray_step<RADIO>(P, R);
+    ray_result(R, out);
 }
 
 // blackholes/src/kerr.hpp:184-203 for one photon (lam, eta, E, y[6])

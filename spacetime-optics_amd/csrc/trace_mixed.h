@@ -304,135 +304,172 @@ struct MixedParams {
     bhmlp::SurrogateTable S;
 };
 
+// Per-ray state of the mixed-policy tracer.
+struct RayStateM {
+    RayConst c;
+    State s;
+    double h, acc0, acc1, acc2, acc3, trans;
+    float psi, ppx, ppy, ppz;
+    int ncross, accepted, flow_steps, it, fate, fp64_attempts;
+    bool done, k1_f32;
+};
+
+template <bool RADIO>
+__device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedParams& M, double x, double yv,
+                                           RayStateM& R) {
+    bhd::camera_photon(P, x, yv, R.c, R.s);
+    R.k1_f32 = R.s.y0 >= M.r_far;
+    if (R.k1_f32) {
+        const RConst<float> cf = make_const<float>(P, R.c);
+        Rhs<float> k = rhs<float>(cf, (float)R.s.y0, (float)R.s.y2);
+        R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
+    } else {
+        const RConst<double> cd = make_const<double>(P, R.c);
+        Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
+        R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
+    }
+    R.h = bhd::initial_step(R.s);
+    R.acc0 = R.acc1 = R.acc2 = R.acc3 = 0.0;
+    R.trans = 1.0;
+    R.psi = 0.f;
+    R.ncross = R.accepted = R.flow_steps = R.it = R.fate = R.fp64_attempts = 0;
+    R.ppx = R.ppy = R.ppz = 0.f;
+    if (RADIO) {
+        float sp = __builtin_sqrtf(fmaxf(1.f - (float)(R.s.y2 * R.s.y2), 0.f));
+        float ph = (float)R.s.y4;
+        R.ppx = sp * __cosf(ph);
+        R.ppy = sp * __sinf(ph);
+        R.ppz = (float)R.s.y2;
+    }
+    R.done = P.max_steps <= 0;
+}
+
+// One attempt of trace.cuh:81-163 under the mixed policy.
+template <bool RADIO>
+__device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedParams& M, RayStateM& R) {
+    const bool flow = RADIO && P.flow_on;
+    State n;
+    double hc = R.h;
+    if (flow && R.s.y0 < P.r_flow_max) {
+        double sn = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0));
+        double cap = P.step_frac * fmin(R.s.y0 * rcpd(fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(R.s.y3) + 1e-30));
+        hc = fmin(hc, cap);
+    }
+    double err;
+    if (R.s.y0 >= M.r_far) {
+        const RConst<float> cf = make_const<float>(P, R.c);
+        err = dp54<float>(cf, R.s, hc, M.tol_far, M.tol_far * 4.0, n);
+    } else {
+        const RConst<double> cd = make_const<double>(P, R.c);
+        if (R.k1_f32) {   // entering the FP64 region: FP64 derivative at the current state
+            Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
+            R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
+        }
+        err = dp54<double>(cd, R.s, hc, P.tol, P.tol4, n);
+        ++R.fp64_attempts;
+    }
+    R.k1_f32 = R.s.y0 >= M.r_far;
+    ++R.it;
+    if (!(err <= 1.0)) {
+        R.h = hc * fmax(0.2, 0.9 * pow_m02_fast(err));
+        if (!(err == err)) R.h = hc * 0.1;
+        if (R.it >= P.max_steps) R.done = true;
+        return;
+    }
+    ++R.accepted;
+    if (RADIO) {
+        float sn = __builtin_sqrtf(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
+        float ph = (float)n.y4;
+        float pnx = sn * __cosf(ph), pny = sn * __sinf(ph), pnz = (float)n.y2;
+        float cx = R.ppy * pnz - R.ppz * pny, cy = R.ppz * pnx - R.ppx * pnz, cz = R.ppx * pny - R.ppy * pnx;
+        R.psi += atan2f(__builtin_sqrtf(cx * cx + cy * cy + cz * cz), R.ppx * pnx + R.ppy * pny + R.ppz * pnz);
+        R.ppx = pnx;
+        R.ppy = pny;
+        R.ppz = pnz;
+    }
+    if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
+    if (flow && n.y0 < P.r_flow_max) {
+        ++R.flow_steps;
+        FlowSample f;
+        if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
+            float ln_j, ln_a, fj, fa;
+            const float sinb = expf_(f.ln_sinb);
+            if (M.use_surrogate &&
+                surrogate_lookup(M.S, (f.ln_nu - f.ln_nuc) * 0.43429448190325176f, f.ln_th * 0.43429448190325176f,
+                                 asinf(fminf(sinb, 1.f)), &fj, &fa)) {
+                ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
+                ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
+            } else {
+                synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a);
+            }
+            double dI, dtau;
+            transfer(M.E, f, ln_j, ln_a, &dI, &dtau);
+            double cI = R.trans * dI;
+            int ord = (int)(R.psi * 0.31830988618379067f);
+            R.acc0 += cI;
+            if (ord == 1) R.acc1 += cI;
+            else if (ord == 2) R.acc2 += cI;
+            else if (ord >= 3) R.acc3 += cI;
+            R.trans *= (double)expf_(-(float)dtau);
+            if (R.trans < 1e-5) {
+                R.fate = -1;
+                R.done = true;
+                return;
+            }
+        }
+    }
+    R.s = n;
+    bhd::project_radial(P, R.c, R.s);
+    R.h = hc * fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
+    if (R.s.y0 < P.r_stop) {
+        R.fate = -1;
+        R.done = true;
+        return;
+    }
+    if (R.s.y0 > P.r_escape && R.s.y1 > 0.0) {
+        R.fate = 1;
+        R.done = true;
+        return;
+    }
+    if (R.it >= P.max_steps) R.done = true;
+}
+
+// Outputs of a finished ray; returns the number of FP64 step attempts.
+__device__ __forceinline__ int ray_result_m(const RayStateM& R, RayResult& out) {
+    out.y[0] = R.s.y0; out.y[1] = R.s.y1; out.y[2] = R.s.y2; out.y[3] = R.s.y3; out.y[4] = R.s.y4;
+    out.y[5] = R.s.y5;
+    out.th_inf = 0.0;
+    out.ph_inf = 0.0;
+    if (R.fate == 1) {
+        double sq = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 1e-30));
+        double tq = (double)acosf(fmaxf(-1.f, fminf(1.f, (float)R.s.y2)));
+        // acos in FP32 is not enough near the poles: one FP64 Newton step on cos
+        tq = tq + (cos(tq) - R.s.y2) * rcpd(fmax(sin(tq), 1e-30));
+        double dthdr = (-R.s.y3 * rcpd(sq)) * rcpd(R.s.y1), dphdr = R.s.k4 * rcpd(R.s.y1);
+        out.th_inf = tq + R.s.y0 * dthdr;
+        out.ph_inf = R.s.y4 + R.s.y0 * dphdr;
+    }
+    out.I[0] = R.acc0; out.I[1] = R.acc1; out.I[2] = R.acc2; out.I[3] = R.acc3;
+    out.trans = R.trans;
+    out.psi = (double)R.psi;
+    out.fate = R.fate;
+    out.ncross = R.ncross;
+    out.order = (int)(R.psi * 0.31830988618379067f);
+    out.steps = R.it;
+    out.accepted = R.accepted;
+    out.flow_steps = R.flow_steps;
+    return R.fp64_attempts;
+}
+
 /* blackholes/src/trace.cuh:62-165 under the mixed policy. Returns the number of FP64 step attempts
    (near field and flow); out.steps counts all attempts. */
 template <bool RADIO>
 __device__ __forceinline__ int trace_ray_mixed(const TraceParams& P, const MixedParams& M, double x, double yv,
                                                RayResult& out) {
-    RayConst c;
-    State s, n;
-    bhd::camera_photon(P, x, yv, c, s);
-    const RConst<float> cf = make_const<float>(P, c);
-    const RConst<double> cd = make_const<double>(P, c);
-    bool k1_f32 = s.y0 >= M.r_far;
-    if (k1_f32) {
-        Rhs<float> k = rhs<float>(cf, (float)s.y0, (float)s.y2);
-        s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
-    } else {
-        Rhs<double> k = rhs<double>(cd, s.y0, s.y2);
-        s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
-    }
-    double h = bhd::initial_step(s);
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0, trans = 1.0;
-    float psi = 0.f;
-    int ncross = 0, accepted = 0, flow_steps = 0, fate = 0, it = 0, fp64_attempts = 0;
-    double th_inf = 0.0, ph_inf = 0.0;
-    float ppx = 0.f, ppy = 0.f, ppz = 0.f;
-    const bool flow = RADIO && P.flow_on;
-    if (RADIO) {
-        float sp = __builtin_sqrtf(fmaxf(1.f - (float)(s.y2 * s.y2), 0.f));
-        float ph = (float)s.y4;
-        ppx = sp * __cosf(ph);
-        ppy = sp * __sinf(ph);
-        ppz = (float)s.y2;
-    }
-    for (it = 0; it < P.max_steps; ++it) {
-        double hc = h;
-        if (flow && s.y0 < P.r_flow_max) {
-            double sn = sqrt(fmax(1.0 - s.y2 * s.y2, 0.0));
-            double cap = P.step_frac * fmin(s.y0 * rcpd(fabs(s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(s.y3) + 1e-30));
-            hc = fmin(hc, cap);
-        }
-        double err;
-        if (s.y0 >= M.r_far) {
-            err = dp54<float>(cf, s, hc, M.tol_far, M.tol_far * 4.0, n);
-        } else {
-            if (k1_f32) {   // entering the FP64 region: FP64 derivative at the current state
-                Rhs<double> k = rhs<double>(cd, s.y0, s.y2);
-                s.k1 = k.d1; s.k3 = k.d3; s.k4 = k.d4; s.k5 = k.d5;
-            }
-            err = dp54<double>(cd, s, hc, P.tol, P.tol4, n);
-            ++fp64_attempts;
-        }
-        k1_f32 = s.y0 >= M.r_far;
-        if (!(err <= 1.0)) {
-            h = hc * fmax(0.2, 0.9 * pow_m02_fast(err));
-            if (!(err == err)) h = hc * 0.1;
-            continue;
-        }
-        ++accepted;
-        if (RADIO) {
-            float sn = __builtin_sqrtf(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
-            float ph = (float)n.y4;
-            float pnx = sn * __cosf(ph), pny = sn * __sinf(ph), pnz = (float)n.y2;
-            float cx = ppy * pnz - ppz * pny, cy = ppz * pnx - ppx * pnz, cz = ppx * pny - ppy * pnx;
-            psi += atan2f(__builtin_sqrtf(cx * cx + cy * cy + cz * cz), ppx * pnx + ppy * pny + ppz * pnz);
-            ppx = pnx;
-            ppy = pny;
-            ppz = pnz;
-        }
-        if ((s.y2 > 0.0) != (n.y2 > 0.0)) ++ncross;
-        if (flow && n.y0 < P.r_flow_max) {
-            ++flow_steps;
-            FlowSample f;
-            if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, c.lam, c.E, hc, f)) {
-                float ln_j, ln_a, fj, fa;
-                const float sinb = expf_(f.ln_sinb);
-                if (M.use_surrogate &&
-                    surrogate_lookup(M.S, (f.ln_nu - f.ln_nuc) * 0.43429448190325176f, f.ln_th * 0.43429448190325176f,
-                                     asinf(fminf(sinb, 1.f)), &fj, &fa)) {
-                    ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
-                    ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
-                } else {
-                    synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a);
-                }
-                double dI, dtau;
-                transfer(M.E, f, ln_j, ln_a, &dI, &dtau);
-                double cI = trans * dI;
-                int ord = (int)(psi * 0.31830988618379067f);
-                acc0 += cI;
-                if (ord == 1) acc1 += cI;
-                else if (ord == 2) acc2 += cI;
-                else if (ord >= 3) acc3 += cI;
-                trans *= (double)expf_(-(float)dtau);
-                if (trans < 1e-5) {
-                    fate = -1;
-                    break;
-                }
-            }
-        }
-        s = n;
-        bhd::project_radial(P, c, s);
-        h = hc * fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
-        if (s.y0 < P.r_stop) {
-            fate = -1;
-            break;
-        }
-        if (s.y0 > P.r_escape && s.y1 > 0.0) {
-            fate = 1;
-            double sq = sqrt(fmax(1.0 - s.y2 * s.y2, 1e-30));
-            double tq = (double)acosf(fmaxf(-1.f, fminf(1.f, (float)s.y2)));
-            // acos in FP32 is not enough near the poles: refine with one FP64 Newton step on cos
-            tq = tq + (cos(tq) - s.y2) * rcpd(fmax(sin(tq), 1e-30));
-            double dthdr = (-s.y3 * rcpd(sq)) * rcpd(s.y1), dphdr = s.k4 * rcpd(s.y1);
-            th_inf = tq + s.y0 * dthdr;
-            ph_inf = s.y4 + s.y0 * dphdr;
-            break;
-        }
-    }
-    out.y[0] = s.y0; out.y[1] = s.y1; out.y[2] = s.y2; out.y[3] = s.y3; out.y[4] = s.y4; out.y[5] = s.y5;
-    out.th_inf = th_inf;
-    out.ph_inf = ph_inf;
-    out.I[0] = acc0; out.I[1] = acc1; out.I[2] = acc2; out.I[3] = acc3;
-    out.trans = trans;
-    out.psi = (double)psi;
-    out.fate = fate;
-    out.ncross = ncross;
-    out.order = (int)(psi * 0.31830988618379067f);
-    out.steps = it < P.max_steps ? it + 1 : P.max_steps;
-    out.accepted = accepted;
-    out.flow_steps = flow_steps;
-    return fp64_attempts;
+    RayStateM R;
+    ray_init_m<RADIO>(P, M, x, yv, R);
+    while (!R.done) ray_step_m<RADIO>(P, M, R);
+    return ray_result_m(R, out);
 }
 
 #pragma clang fp contract(off)

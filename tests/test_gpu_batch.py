@@ -1,0 +1,158 @@
+"""Persistent batched tracer (bh_trace_batch, bh_autotune) against the one-frame-per-launch path.
+
+The persistent kernel runs the same per-ray integrator (init / step / result) as bh_render, only
+scheduled differently (work queues, lane retirement and refill, graph replay), so every pixel must
+be bit-identical to bh_render's for both precisions, for every launch configuration, and for
+batches that mix frame sizes, spins and turbulence time slices. Frame totals (flux, step counters)
+must agree: integer counters exactly, the flux sum to 1e-12 relative (summation order).
+"""
+import numpy as np
+import pytest
+
+import bhrt_amd as bh
+from helpers import D2R, setup, shot, tracer_for
+
+pytestmark = pytest.mark.gpu
+
+
+def frame_for(S, ini, W, H, t=0.0, sets=""):
+    r = float(bh.config_str(ini, "camera.r", sets))
+    incl = float(bh.config_str(ini, "camera.incl", sets)) * D2R
+    fov = float(bh.config_str(ini, "camera.fov_m", sets))
+    F = bh.make_frame(S.a, r, incl, fov=fov, W=W, H=H, spp=1, seed=0x9E3779B9)
+    F.t = t
+    return F
+
+
+def render_each(t, frames, opts):
+    """bh_render of each (Scene, Frame) in turn on the same context."""
+    imgs, flux, cnt = [], 0.0, np.zeros(5, dtype=np.int64)
+    for S, F in frames:
+        t.update_scene(S)
+        img, st = t.render(F, opts)
+        imgs.append(img)
+        flux += st.flux_sum
+        cnt += [st.step_attempts, st.steps_accepted, st.flow_steps, st.step_limited, st.fp64_steps]
+    return imgs, flux, cnt
+
+
+def assert_same(imgs_a, imgs_b, what):
+    assert len(imgs_a) == len(imgs_b)
+    for i, (a, b) in enumerate(zip(imgs_a, imgs_b)):
+        assert a.shape == b.shape, (what, i)
+        bad = np.argwhere((a.view(np.int32) != b.view(np.int32)).any(axis=-1))
+        assert len(bad) == 0, f"{what}: frame {i} differs at {len(bad)} pixels, first {bad[:3].tolist()}"
+
+
+def stats_vec(st):
+    return np.array([st.step_attempts, st.steps_accepted, st.flow_steps, st.step_limited, st.fp64_steps])
+
+
+@pytest.fixture(scope="module")
+def c2():
+    cfg = setup(shot("c2_radio_1024.ini"), W=256, H=256)
+    t = tracer_for(cfg)
+    yield cfg, t
+    t.close()
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_batch_single_frame_every_config(c2, prec):
+    cfg, t = c2
+    o = t.opts(precision=prec)
+    want, flux, cnt = render_each(t, [(cfg["S"], cfg["F"])], o)
+    for block, gpc, interval, refill, graph in [(256, 2, 8, 16, 1), (256, 2, 8, 16, 0), (64, 1, 1, 1, 0),
+                                                (128, 4, 32, 48, 1), (256, 8, 3, 64, 1)]:
+        got, st = t.trace_batch([(cfg["S"], cfg["F"])], o, block=block, grid_per_cu=gpc, interval=interval,
+                                refill_min=refill, use_graph=graph)
+        assert_same(got, want, f"prec {prec} cfg {(block, gpc, interval, refill, graph)}")
+        assert np.array_equal(stats_vec(st), cnt)
+        assert st.samples == 256 * 256
+        assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_batch_mixed_sizes_and_spins(c2, prec):
+    """One launch over frames of different size (ragged, not multiples of the 8 x 8 queue tile) and spin."""
+    cfg, t = c2
+    ini = shot("c2_radio_1024.ini")
+    frames = []
+    for spin, W, H in [(0.94, 256, 256), (0.5, 130, 97), (0.1, 64, 200), (0.99, 8, 8), (0.94, 1, 3)]:
+        S, *_ = bh.build_scene(ini, f"bh.spin={spin}")
+        frames.append((S, frame_for(S, ini, W, H)))
+    o = t.opts(precision=prec)
+    want, flux, cnt = render_each(t, frames, o)
+    t.update_scene(cfg["S"])
+    got, st = t.trace_batch(frames, o)
+    assert_same(got, want, f"batch prec {prec}")
+    assert np.array_equal(stats_vec(st), cnt)
+    assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
+
+
+def test_batch_turbulence_time_slices():
+    """Frames at different times read different turbulence slices (linear in time between slices)."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=96, H=96)
+    grid = bh.turbulence(64, 4, 60.0, 0.3, 5.0, 30.0, 10.0, 3)
+    t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], grid=grid, turb_half=60.0, turb_t0=0.0,
+                  turb_dt=10.0)
+    ini = shot("c2_radio_1024.ini")
+    frames = [(cfg["S"], frame_for(cfg["S"], ini, 96, 96, t=tt)) for tt in (0.0, 4.0, 15.0, 29.0)]
+    o = t.opts(precision=bh.PREC_FP64)
+    want, _, _ = render_each(t, frames, o)
+    got, _ = t.trace_batch(frames, o)
+    assert_same(got, want, "time slices")
+    assert not np.array_equal(want[0], want[2])   # the slices differ
+    t.close()
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_batch_vacuum(prec):
+    cfg = setup(shot("c1_vacuum_256.ini"))
+    t = tracer_for(cfg)
+    o = t.opts(precision=prec)
+    want, _, cnt = render_each(t, [(cfg["S"], cfg["F"])], o)
+    got, st = t.trace_batch([(cfg["S"], cfg["F"])] * 3, o)
+    assert_same(got, want * 3, f"vacuum prec {prec}")
+    assert np.array_equal(stats_vec(st), 3 * cnt)
+    t.close()
+
+
+def test_graph_replay_with_eager_work_between(c2):
+    cfg, t = c2
+    o = t.opts(precision=bh.PREC_MIXED)
+    frames = [(cfg["S"], cfg["F"])] * 2
+    first, st0 = t.trace_batch(frames, o)
+    for _ in range(4):
+        t.render(cfg["F"], t.opts())   # eager launches between replays
+        again, st = t.trace_batch(frames, o)
+        assert_same(again, first, "replay")
+        assert np.array_equal(stats_vec(st), stats_vec(st0))
+        assert st.flux_sum == st0.flux_sum
+
+
+def test_autotune(c2):
+    cfg = setup(shot("c2_radio_1024.ini"), W=128, H=128)
+    t = tracer_for(cfg)
+    frames = [(cfg["S"], cfg["F"])]
+    o = t.opts(precision=bh.PREC_MIXED)
+    want, _ = t.trace_batch(frames, o, block=256, grid_per_cu=2, interval=8, refill_min=16)
+    best, ms, tried = t.autotune(frames, o)
+    assert len(tried) >= 10 and ms > 0
+    assert ms <= tried[:, 4].min() + 1e-9
+    assert best.block in (64, 128, 256) and 1 <= best.interval <= 64
+    got, _ = t.trace_batch(frames, o)   # no configuration given: the tuned one
+    assert_same(got, want, "tuned")
+    t.close()
+
+
+def test_batch_errors(c2):
+    cfg, t = c2
+    v = setup(shot("c1_vacuum_256.ini"))
+    with pytest.raises(bh.BhError, match="share the render mode"):
+        t.trace_batch([(cfg["S"], cfg["F"]), (v["S"], v["F"])])
+    with pytest.raises(bh.BhError, match="block must be"):
+        t.trace_batch([(cfg["S"], cfg["F"])], block=96)
+    with pytest.raises(bh.BhError, match="empty batch"):
+        t.trace_batch([])
+    with pytest.raises(bh.BhError, match="MLP emissivity"):
+        t.trace_batch([(cfg["S"], cfg["F"])], t.opts(precision=bh.PREC_FP64, emissivity=bh.EMIS_MLP))

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import bhrt_amd as bh
-from helpers import D2R, pixel_grid, setup, shot, tracer_for
+from helpers import D2R, pixel_grid, scene_dict, setup, shot, tracer_for
+from oracle_ctypes import make_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -129,6 +130,28 @@ def test_radio_frame_flux_256(oracle_bhm):
     assert st.step_attempts == rays["steps"].sum()
     assert st.flow_steps == rays["flow_steps"].sum()
     assert st.step_limited == (rays["fate"] == 0).sum()
+
+
+def test_radio_time_slices(oracle_bhm):
+    """Time-sliced turbulence (4 slices 10 M apart): a 64^2 tile at frame times inside, between
+    and beyond the slices, per-pixel equal to the oracle."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=256, H=256)
+    grid = bh.turbulence(64, 4, 60.0, 0.3, 5.0, 30.0, 10.0, 3)
+    t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], grid=grid, turb_half=60.0,
+                  turb_t0=-20.0, turb_dt=10.0)
+    OS = make_scene(scene_dict(cfg["S"]), k2=cfg["k2"], grid=grid, turb_half=60.0, turb_t0=-20.0, turb_dt=10.0)
+    OS.k2_l0 = cfg["l0"]
+    OS.k2_dl = cfg["dl"]
+    px, py = pixel_grid(64, 64, 96, 96)
+    for tf in (-35.0, 0.0, 3.7, 50.0):
+        F = bh.Frame.from_buffer_copy(cfg["F"])
+        F.t = tf
+        OF = cfg["OF"]
+        OF.t = tf
+        gpu = t.trace_tile(F, 96, 96, 64, 64)
+        _, ref, _ = oracle_bhm.trace_pixels(OS, OF, px, py, spp=1, jitter=False)
+        assert_rays_equal(gpu, ref, f"t = {tf}")
+    OF.t = 0.0
 
 
 def test_radio_flux_jittered(oracle_bhm):

@@ -128,3 +128,32 @@ def test_ref_library_live_when_present(golden):
         a, lam, eta, y, k1, h, tol = row[0], row[1], row[2], row[3:9], row[9:15], row[15], row[16]
         err, yo, k7 = ref.dp54_step(a, lam, eta, y, k1, h, tol)
         assert bits_equal(np.concatenate([[err], yo, k7]), want)
+
+
+def test_turbulence_lookup_closed_forms(oracle_bhm):
+    """The oracle's turbulence lookup against what the interpolation must reproduce: a grid that is
+    affine in (x, y, z) and affine across time slices is returned exactly (up to rounding) at any
+    point and time inside the grid, and times outside the slices clamp to the first/last slice."""
+    from oracle_ctypes import make_scene
+    n, nt, half, t0, dt = 16, 3, 10.0, 5.0, 2.0
+    dx = 2 * half / n
+    c = (np.arange(n) + 0.5) * dx - half   # cell centres
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    slope = np.array([0.3, -0.2, 0.1])
+    base = 2.0 + slope[0] * x + slope[1] * y + slope[2] * z
+    grid = np.stack([base + 0.5 * k for k in range(nt)]).astype(np.float64)
+    S = make_scene({"turb_n": n}, grid=grid.astype(np.float32), turb_half=half, turb_t0=t0, turb_dt=dt)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        p = rng.uniform(-half + dx, half - dx, 3)
+        r = float(np.linalg.norm(p))
+        u = p[2] / r
+        phi = math.atan2(p[1], p[0])
+        t = rng.uniform(t0 - 3, t0 + (nt - 1) * dt + 3)
+        k = min(max((t - t0) / dt, 0.0), nt - 1.0)
+        want = 2.0 + slope @ p + 0.5 * k
+        got = oracle_bhm.turb_factor(S, r, u, phi, t)
+        assert abs(got - want) < 1e-5, (p, t, got, want)
+    # single slice: time is ignored
+    S1 = make_scene({"turb_n": n}, grid=grid[0].astype(np.float32), turb_half=half)
+    assert oracle_bhm.turb_factor(S1, 3.0, 0.2, 0.7, 0.0) == oracle_bhm.turb_factor(S1, 3.0, 0.2, 0.7, 99.0)
