@@ -7,7 +7,11 @@ grid already resident in HBM; the frame stays in device memory (bh_render_reside
 JSON line. The CPU baseline is the oracle's glibc build (the reference integrator's arithmetic,
 restated in C++) on a bounded random pixel sample of the same frame, over all host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes fp64,mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes fp64,mixed,...] [--autotune]
+
+Modes: fp64 / mixed / mixed_mlp run one launch per frame (bh_render_resident); p_* run the same
+tracers through the persistent batched kernel (bh_trace_batch, graph replay). `value` is the
+fastest mode whose frame is within 0.1% flux and 1% median per-pixel error of the FP64 frame.
 """
 import argparse
 import json
@@ -31,40 +35,67 @@ WORKLOAD = ("c2_radio_1024: 1024x1024 Kerr frame at 230 GHz, a=0.94, i=17 deg, r
 # Algorithmic work per unit (one DP54 step attempt; extra work of an accepted step inside the flow)
 FLOP_PER_ATTEMPT = 600.0
 FLOP_PER_FLOW_STEP = 200.0
-FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 vector peak
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def bench_mode(tracer, F, opts, steps, warmup):
-    for _ in range(warmup):
-        tracer.render_resident(F, opts)
-    kernel_ms, stats = [], None
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        _, st = tracer.render_resident(F, opts)   # synchronous: returns after the frame is done
-        kernel_ms.append(st.ms_kernel)
-        stats = st
-    wall = time.perf_counter() - t0
-    rays = F.W * F.H
-    work = FLOP_PER_ATTEMPT * stats.step_attempts + FLOP_PER_FLOW_STEP * stats.flow_steps
+def summarise(kernel_ms, wall, steps, rays, stats, prec):
+    """Throughput, counters and the roofline position of one mode. Work is counted in flop-units
+    per DP54 step attempt and per in-flow step; attempts the kernel ran in FP32 (mixed far field)
+    and the mixed tracer's FP32 log-space emission are priced at the FP32 vector peak, the rest at
+    the FP64 peak, so the roof is the time the same work would take at the two peaks."""
     avg_kernel_s = float(np.mean(kernel_ms)) * 1e-3
+    att = stats.step_attempts
+    a64 = stats.fp64_steps if prec == bh.PREC_MIXED else att
+    w64 = FLOP_PER_ATTEMPT * a64 + (FLOP_PER_FLOW_STEP * stats.flow_steps if prec == bh.PREC_FP64 else 0.0)
+    w32 = FLOP_PER_ATTEMPT * (att - a64) + (FLOP_PER_FLOW_STEP * stats.flow_steps if prec == bh.PREC_MIXED else 0.0)
+    t_roof = w64 / (FP64_PEAK_TFLOPS * 1e12) + w32 / (FP32_PEAK_TFLOPS * 1e12)
     return {
         "mrays_per_s": rays * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
         "kernel_ms": float(np.mean(kernel_ms)),
         "kernel_mrays_per_s": rays / avg_kernel_s / 1e6,
-        "step_attempts": int(stats.step_attempts),
-        "steps_per_ray": stats.step_attempts / rays,
+        "step_attempts": int(att),
+        "steps_per_ray": att / rays,
         "flow_steps": int(stats.flow_steps),
-        "fp64_attempts": int(stats.fp64_steps),
+        "fp64_attempts": int(a64),
         "step_limited": int(stats.step_limited),
         "flux_sum": stats.flux_sum,
-        "work_flop": work,
-        "achieved_tflops": work / avg_kernel_s / 1e12,
+        "work_flop": w64 + w32,
+        "achieved_tflops": (w64 + w32) / avg_kernel_s / 1e12,
+        "peak_tflops": (w64 + w32) / t_roof / 1e12,
+        "roof_frac": t_roof / avg_kernel_s,
     }
+
+
+def bench_mode(tracer, F, S, opts, steps, warmup, persistent):
+    """Times `steps` full-frame traces; persistent: one-frame bh_trace_batch (graph replay),
+    else bh_render_resident. Both return after the frame is done."""
+    run = ((lambda: tracer.trace_batch([(S, F)], opts, resident=True)) if persistent
+           else (lambda: tracer.render_resident(F, opts)))
+    for _ in range(warmup):
+        run()
+    kernel_ms, stats = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = run()
+        kernel_ms.append(st.ms_kernel)
+        stats = st
+    wall = time.perf_counter() - t0
+    return summarise(kernel_ms, wall, steps, F.W * F.H, stats, opts.precision)
+
+
+def frame_error(img, ref):
+    """Flux and per-pixel errors of an image against the FP64 image (pixels above 1e-3 of the peak)."""
+    Ia, Ib = img[..., 0].astype(np.float64), ref[..., 0].astype(np.float64)
+    mask = Ib > 1e-3 * Ib.max()
+    rel = np.abs(Ia[mask] - Ib[mask]) / Ib[mask]
+    return {"flux_rel": float(abs(Ia.sum() - Ib.sum()) / Ib.sum()), "pixel_median_rel": float(np.median(rel)),
+            "pixel_p99_rel": float(np.percentile(rel, 99)), "pixels": int(mask.sum())}
 
 
 def cpu_baseline(S, k2, l0, dl, F, n_pix, seed=11):
@@ -102,6 +133,17 @@ def cpu_baseline(S, k2, l0, dl, F, n_pix, seed=11):
                       f"mean {rays['steps'].mean():.1f} DP54 steps/ray"}
 
 
+MODES = {   # name: (precision, emissivity, persistent)
+    "fp64": (bh.PREC_FP64, bh.EMIS_EXACT, False),
+    "mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, False),
+    "mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, False),
+    "p_fp64": (bh.PREC_FP64, bh.EMIS_EXACT, True),
+    "p_mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, True),
+    "p_mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, True),
+}
+FLUX_GATE, PIXEL_GATE = 1e-3, 1e-2   # a mode counts only below 0.1% flux and 1% median per-pixel error
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +151,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pixels", type=int, default=1048576)
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default=",".join(MODES))
+    ap.add_argument("--autotune", action="store_true", help="autotune the persistent launch before timing it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (rays shard with no "
@@ -118,6 +161,11 @@ def main():
     if bh.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    modes = [m for m in args.modes.split(",") if m]
+    for m in modes:
+        if m not in MODES:
+            print(f"bench.py: unknown mode {m}; known: {', '.join(MODES)}", file=sys.stderr)
+            sys.exit(2)
 
     log("bench: building scene, 128^3 turbulence, flux normalisation to 0.6 Jy")
     tracer, S, F, opts, jy = bh.Tracer.from_ini(INI, normalize=True)
@@ -125,36 +173,53 @@ def main():
     _, k2, (l0, dl), _, _ = bh.build_scene(INI)
     log(f"bench: n0 = {S.n0:.6g} cm^-3, flux {jy:.4f} Jy")
 
+    setup = {}
+    if any(MODES[m][1] == bh.EMIS_MLP for m in modes):
+        t0 = time.perf_counter()
+        rep = tracer.mlp_train()
+        t1 = time.perf_counter()
+        bake = tracer.mlp_bake()
+        t2 = time.perf_counter()
+        setup["mlp_train_s"] = round(t1 - t0, 3)
+        setup["mlp_bake_s"] = round(t2 - t1, 3)
+        setup["mlp_loss_first_last"] = [rep["loss_first"], rep["loss_last"]]
+        setup["mlp_table_cells"] = bake["cells"]
+        setup["mlp_hot_cells"] = bake["hot_cells"]
+        setup["mlp_failed_hot_cells"] = bake["failed_hot_cells"]
+        log(f"bench: surrogate trained in {t1 - t0:.2f} s, baked in {t2 - t1:.2f} s: {setup}")
+    if args.autotune and any(MODES[m][2] for m in modes):
+        o = bh.Tracer.opts(precision=bh.PREC_MIXED)
+        best, ms, tried = tracer.autotune([(S, F)], o)
+        setup["autotuned"] = best.as_dict()
+        setup["autotune_configs"] = len(tried)
+        log(f"bench: autotuned persistent launch {best.as_dict()} ({ms:.2f} ms, {len(tried)} configs)")
+
     results = {}
-    modes = [m for m in args.modes.split(",") if m]
     for m in modes:
-        o = bh.Tracer.opts(precision=bh.PREC_FP64 if m == "fp64" else bh.PREC_MIXED,
-                           emissivity=bh.EMIS_MLP if m == "mixed_mlp" else bh.EMIS_EXACT, jitter=0, spp=1)
+        prec, emis, persistent = MODES[m]
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1)
         try:
-            results[m] = bench_mode(tracer, F, o, args.steps, args.warmup)
+            results[m] = bench_mode(tracer, F, S, o, args.steps, args.warmup, persistent)
             log(f"bench: {m}: {results[m]['mrays_per_s']:.3f} Mrays/s, kernel {results[m]['kernel_ms']:.2f} ms, "
-                f"{results[m]['achieved_tflops']:.2f} TFLOP/s")
+                f"{results[m]['achieved_tflops']:.2f} TFLOP/s ({100 * results[m]['roof_frac']:.1f}% of roof)")
         except bh.BhError as e:
             log(f"bench: {m}: not available ({e})")
 
-    # accuracy of the mixed tracer against the FP64 tracer (bit-identical to the oracle) on this frame
-    pix_err = None
-    if "mixed" in results:
-        ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1))
-        mix, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_EXACT, jitter=0, spp=1))
-        Ia, Ib = mix[..., 0].astype(np.float64), ref[..., 0].astype(np.float64)
-        mask = Ib > 1e-3 * Ib.max()
-        rel = np.abs(Ia[mask] - Ib[mask]) / Ib[mask]
-        pix_err = {"flux_rel": abs(Ia.sum() - Ib.sum()) / Ib.sum(), "pixel_median_rel": float(np.median(rel)),
-                   "pixel_p99_rel": float(np.percentile(rel, 99)), "pixels": int(mask.sum())}
-        log(f"bench: mixed vs fp64 frame: {pix_err}")
+    # accuracy of every mode against the FP64 tracer (bit-identical to the oracle) on this frame
+    ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1))
+    for m, r in results.items():
+        prec, emis, persistent = MODES[m]
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1)
+        img = tracer.trace_batch([(S, F)], o)[0][0] if persistent else tracer.render(F, o)[0]
+        r["error_vs_fp64"] = frame_error(img, ref)
+        r["passes_gate"] = bool(r["error_vs_fp64"]["flux_rel"] < FLUX_GATE and
+                                r["error_vs_fp64"]["pixel_median_rel"] < PIXEL_GATE)
+        log(f"bench: {m} vs fp64: {r['error_vs_fp64']}")
 
-    head = results.get("mixed") or results["fp64"]
-    head_mode = "mixed" if "mixed" in results else "fp64"
-    fp64 = results["fp64"]
-    flux_err = None
-    if "mixed" in results:
-        flux_err = abs(results["mixed"]["flux_sum"] - fp64["flux_sum"]) / abs(fp64["flux_sum"])
+    passing = [m for m in results if results[m]["passes_gate"]]
+    head_mode = max(passing, key=lambda m: results[m]["mrays_per_s"])
+    head = results[head_mode]
+    fp64 = results.get("fp64")
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_c2.json")
@@ -169,7 +234,7 @@ def main():
         log(f"bench: CPU baseline on {args.cpu_pixels} pixels")
         cpu = cpu_baseline(S, k2, l0, dl, F, args.cpu_pixels)
 
-    achieved = head["achieved_tflops"]
+    prec = MODES[head_mode][0]
     line = {
         "metric": "Mrays/s, 1024^2 230GHz Kerr frame: plain FP64 vs mixed+MLP at <0.1% flux err",
         "value": round(head["mrays_per_s"], 4),
@@ -181,22 +246,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head_mode == "fp64" else "f32/f64 + bf16 MLP",
+        "dtype": "f64" if prec == bh.PREC_FP64 else ("f32/f64 + bf16-MLP table" if MODES[head_mode][1] else "f32/f64"),
         "data": "synthetic",
-        "config": {"workload": WORKLOAD, "precision": head_mode, "rays_per_step": F.W * F.H,
-                   "n0_cm3": S.n0, "flux_jy": jy},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "note": "FP64 vector ALU roof; algorithmic work = 600 flop-units per DP54 step attempt "
-                             "+ 200 per in-flow step, counted by the kernel"},
+        "config": {"workload": WORKLOAD, "mode": head_mode, "rays_per_step": F.W * F.H, "n0_cm3": S.n0,
+                   "flux_jy": jy},
+        "roofline": {"bound": "valu", "achieved": round(head["achieved_tflops"], 3),
+                     "peak": round(head["peak_tflops"], 2), "unit": "TFLOP/s", "frac": round(head["roof_frac"], 4),
+                     "traffic": traffic,
+                     "note": "vector-ALU roof (FP64 78.6, FP32 157.3 TFLOP/s, weighted by the work each precision "
+                             "did); work = 600 flop-units per DP54 step attempt + 200 per in-flow step, counted "
+                             "by the kernel; HIP-event kernel time"},
         "cpu_baseline": cpu,
-        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in fp64.items()},
+        "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for m, r in results.items()},
+        "setup": setup,
     }
-    if "mixed" in results:
-        line["mixed"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in results["mixed"].items()}
-        line["speedup_mixed_vs_fp64"] = round(results["mixed"]["mrays_per_s"] / fp64["mrays_per_s"], 3)
-        line["flux_rel_err_mixed_vs_fp64"] = flux_err
-        line["mixed_vs_fp64_frame_error"] = pix_err
+    if fp64:
+        line["speedup_vs_fp64"] = round(head["mrays_per_s"] / fp64["mrays_per_s"], 3)
     if cpu:
         line["gpu_vs_cpu"] = round(head["mrays_per_s"] / cpu["value"], 1)
     tracer.close()

@@ -281,6 +281,38 @@ def selftest(device=0):
 
 
 # --- device context ---------------------------------------------------------------------------------
+def batch_from_ini(ini, sets="", n0=None):
+    """(Scene, Frame) list of a [batch] shot: frame i has spin spins[i % len(spins)], size
+    sizes[(i // len(spins)) % len(sizes)] squared and time i * frame_dt_m; camera from [camera].
+    n0 (the normalised density, e.g. Tracer.scene().n0) replaces every scene's n0 when given."""
+    import math
+    def floats(key, default):
+        v = config_str(ini, key, sets)
+        return [float(x) for x in v.split(",")] if v else default
+    nf = int(config_str(ini, "batch.frames", sets) or 1)
+    spins = floats("batch.spins", [float(config_str(ini, "bh.spin", sets) or 0.94)])
+    sizes = [int(x) for x in floats("batch.sizes", [float(config_str(ini, "camera.width", sets))])]
+    dt = float(config_str(ini, "batch.frame_dt_m", sets) or 0.0)
+    r = float(config_str(ini, "camera.r", sets))
+    incl = float(config_str(ini, "camera.incl", sets)) * math.pi / 180
+    fov = float(config_str(ini, "camera.fov_m", sets))
+    scenes = {}
+    out = []
+    for i in range(nf):
+        a = spins[i % len(spins)]
+        if a not in scenes:
+            extra = f"bh.spin={a!r}"
+            scenes[a] = build_scene(ini, f"{sets};{extra}" if sets else extra)[0]
+            if n0 is not None:
+                scenes[a].n0 = n0
+        S = scenes[a]
+        n = sizes[(i // len(spins)) % len(sizes)]
+        F = make_frame(S.a, r, incl, fov=fov, W=n, H=n, spp=1, seed=(0x9E3779B9 * (i + 1)) & 0xFFFFFFFF)
+        F.t = i * dt
+        out.append((S, F))
+    return out
+
+
 class Tracer:
     """A device context (bh_ctx)."""
 
@@ -406,19 +438,23 @@ class Tracer:
         return arr
 
     def trace_batch(self, frames, opts=None, cfg=None, block=0, grid_per_cu=0, interval=0, refill_min=0,
-                    use_graph=1):
-        """frames: list of (Scene, Frame). Returns ([H x W x 4 images], Stats). cfg None and all launch
-        arguments 0 -> the last autotuned configuration, else the defaults."""
+                    use_graph=1, resident=False):
+        """frames: list of (Scene, Frame). Returns ([H x W x 4 images], Stats), or (None, Stats) with
+        resident=True (the images stay in device memory). cfg None and all launch arguments 0 -> the
+        last autotuned configuration, else the defaults."""
         arr = self._batch(frames)
         npix = sum(F.W * F.H for _, F in frames)
-        px = np.zeros((npix, 4), dtype=np.float32)
+        px = np.zeros((0 if resident else npix, 4), dtype=np.float32)
         st = Stats()
         o = opts or self.opts()
         if cfg is None and (block or grid_per_cu or interval or refill_min or use_graph != 1):
             cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph)
         pc = C.byref(cfg) if cfg is not None else None
-        self._check(self.L.bh_trace_batch(self.ctx, arr, len(frames), C.byref(o), pc, px.ctypes.data_as(_F),
-                                          C.byref(st)), "bh_trace_batch")
+        dst = None if resident else px.ctypes.data_as(_F)
+        self._check(self.L.bh_trace_batch(self.ctx, arr, len(frames), C.byref(o), pc, dst, C.byref(st)),
+                    "bh_trace_batch")
+        if resident:
+            return None, st
         out, off = [], 0
         for _, F in frames:
             out.append(px[off:off + F.W * F.H].reshape(F.H, F.W, 4))

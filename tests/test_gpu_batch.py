@@ -156,3 +156,21 @@ def test_batch_errors(c2):
         t.trace_batch([])
     with pytest.raises(bh.BhError, match="MLP emissivity"):
         t.trace_batch([(cfg["S"], cfg["F"])], t.opts(precision=bh.PREC_FP64, emissivity=bh.EMIS_MLP))
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_batched_animation_shot_reduced(prec):
+    """The batched-animation shot (time-sliced turbulence, four spins, mixed sizes) at reduced size:
+    one persistent launch equals frame-by-frame rendering."""
+    ini = shot("c5_batch_anim.ini")
+    sets = "batch.frames=8;batch.sizes=40,64;turbulence.grid=32;turbulence.slices=8;turbulence.dt_m=40"
+    t, S, F, _, _ = bh.Tracer.from_ini(ini, sets, normalize=False)
+    frames = bh.batch_from_ini(ini, sets)
+    assert len({round(s.a, 6) for s, _ in frames}) == 4 and len({f.W for _, f in frames}) == 2
+    o = t.opts(precision=prec)
+    want, flux, cnt = render_each(t, frames, o)
+    got, st = t.trace_batch(frames, o)
+    assert_same(got, want, "c5 reduced")
+    assert np.array_equal(stats_vec(st), cnt)
+    assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
+    t.close()
