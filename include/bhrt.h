@@ -172,6 +172,8 @@ typedef struct bh_batch_cfg {
     int interval;       /* integration steps between retire/refill (0 -> 8) */
     int refill_min;     /* free lanes needed to refill (0 -> 16) */
     int use_graph;      /* capture and replay the launch as a hipGraph */
+    int waves;          /* 0: kernel compiled for the compiler's register budget; 3 or 4: compiled for at
+                           least that many resident waves per SIMD (fewer registers) */
 } bh_batch_cfg;
 
 /* Traces every frame of `frames`; px_out (nullable) receives the frames one after the other,
@@ -181,7 +183,7 @@ int bh_trace_batch(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_tr
 
 /* Times bh_trace_batch over a grid of configurations on the given workload and returns the fastest
    (best) with its kernel time (best_ms). tried (nullable, 6 doubles per configuration: block,
-   grid_per_cu, interval, refill_min, ms, 0) receives up to max_tried rows; *n_tried their number. */
+   grid_per_cu, interval, refill_min, ms, waves) receives up to max_tried rows; *n_tried their number. */
 int bh_autotune(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_trace_opts* opts, bh_batch_cfg* best,
                 double* best_ms, double* tried, int max_tried, int* n_tried);
 

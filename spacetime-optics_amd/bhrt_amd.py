@@ -82,7 +82,7 @@ class BatchFrame(C.Structure):
 
 class BatchCfg(C.Structure):
     _fields_ = [("block", C.c_int), ("grid_per_cu", C.c_int), ("interval", C.c_int), ("refill_min", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("waves", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -438,7 +438,7 @@ class Tracer:
         return arr
 
     def trace_batch(self, frames, opts=None, cfg=None, block=0, grid_per_cu=0, interval=0, refill_min=0,
-                    use_graph=1, resident=False):
+                    use_graph=1, waves=0, resident=False):
         """frames: list of (Scene, Frame). Returns ([H x W x 4 images], Stats), or (None, Stats) with
         resident=True (the images stay in device memory). cfg None and all launch arguments 0 -> the
         last autotuned configuration, else the defaults."""
@@ -447,8 +447,8 @@ class Tracer:
         px = np.zeros((0 if resident else npix, 4), dtype=np.float32)
         st = Stats()
         o = opts or self.opts()
-        if cfg is None and (block or grid_per_cu or interval or refill_min or use_graph != 1):
-            cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph)
+        if cfg is None and (block or grid_per_cu or interval or refill_min or waves or use_graph != 1):
+            cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph, waves)
         pc = C.byref(cfg) if cfg is not None else None
         dst = None if resident else px.ctypes.data_as(_F)
         self._check(self.L.bh_trace_batch(self.ctx, arr, len(frames), C.byref(o), pc, dst, C.byref(st)),

@@ -154,8 +154,10 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
     }
 }
 
-template <int PREC, bool RADIO>
-__global__ __launch_bounds__(BLOCK) void k_persist(const bhp::BatchFrame* __restrict__ frames, int nframes,
+// WAVES > 1 asks the compiler for at least that many resident waves per SIMD (fewer VGPRs, possibly
+// spills); 1 leaves the register budget to the compiler.
+template <int PREC, bool RADIO, int WAVES>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_persist(const bhp::BatchFrame* __restrict__ frames, int nframes,
                                                    unsigned int* __restrict__ queue, int interval, int refill_min,
                                                    BlockOut* __restrict__ blk) {
     double flux;
@@ -439,9 +441,18 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
 
 using PersistFn = void (*)(const bhp::BatchFrame*, int, unsigned int*, int, int, BlockOut*);
 
-PersistFn pick_persist(int prec, bool radio) {
-    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true> : k_persist<BH_PREC_FP64, false>;
-    return radio ? k_persist<BH_PREC_MIXED, true> : k_persist<BH_PREC_MIXED, false>;
+template <int W>
+PersistFn pick_persist_w(int prec, bool radio) {
+    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, W> : k_persist<BH_PREC_FP64, false, W>;
+    return radio ? k_persist<BH_PREC_MIXED, true, W> : k_persist<BH_PREC_MIXED, false, W>;
+}
+
+PersistFn pick_persist(int prec, bool radio, int waves) {
+    switch (waves) {
+        case 3: return pick_persist_w<3>(prec, radio);
+        case 4: return pick_persist_w<4>(prec, radio);
+        default: return pick_persist_w<1>(prec, radio);
+    }
 }
 
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
@@ -475,6 +486,7 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     if (cfg.grid_per_cu < 1 || cfg.grid_per_cu > 32) return fail(c, BH_ERR_ARG, "grid_per_cu must be in [1, 32]");
     if (cfg.interval < 1 || cfg.interval > 4096) return fail(c, BH_ERR_ARG, "interval must be in [1, 4096]");
     if (cfg.refill_min < 1 || cfg.refill_min > 64) return fail(c, BH_ERR_ARG, "refill_min must be in [1, 64]");
+    if (cfg.waves != 0 && cfg.waves != 3 && cfg.waves != 4) return fail(c, BH_ERR_ARG, "waves must be 0, 3 or 4");
     const int mode = frames[0].scene.mode;
     long long total = 0;
     std::vector<bhp::BatchFrame> hb((size_t)n);
@@ -517,7 +529,7 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid);
     if (rc) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_bframes, hb.data(), sizeof(bhp::BatchFrame) * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PersistFn k = pick_persist(o.precision, mode == BH_MODE_RADIO);
+    PersistFn k = pick_persist(o.precision, mode == BH_MODE_RADIO, cfg.waves);
     const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
@@ -786,7 +798,7 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
             row[2] = g.interval;
             row[3] = g.refill_min;
             row[4] = s.ms_kernel;
-            row[5] = 0.0;
+            row[5] = g.waves;
         }
         ++count;
         return BH_OK;
@@ -794,14 +806,15 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
     int rc = time_cfg(cur, &cur_ms);
     if (rc) return rc;
     const int blocks[] = {64, 128, 256}, grids[] = {1, 2, 3, 4, 8}, intervals[] = {2, 4, 8, 16, 32, 64},
-              refills[] = {1, 8, 16, 32, 48};
+              refills[] = {1, 8, 16, 32, 48}, waves[] = {0, 3, 4};
     for (int pass = 0; pass < 2; ++pass) {
-        for (int p = 0; p < 4; ++p) {
-            const int* vals = p == 0 ? blocks : p == 1 ? grids : p == 2 ? intervals : refills;
-            const int nv = p == 0 ? 3 : p == 1 ? 5 : p == 2 ? 6 : 5;
+        for (int p = 0; p < 5; ++p) {
+            const int* vals = p == 0 ? waves : p == 1 ? blocks : p == 2 ? grids : p == 3 ? intervals : refills;
+            const int nv = p == 0 ? 3 : p == 1 ? 3 : p == 2 ? 5 : p == 3 ? 6 : 5;
             for (int i = 0; i < nv; ++i) {
                 bh_batch_cfg g = cur;
-                int* field = p == 0 ? &g.block : p == 1 ? &g.grid_per_cu : p == 2 ? &g.interval : &g.refill_min;
+                int* field = p == 0 ? &g.waves : p == 1 ? &g.block : p == 2 ? &g.grid_per_cu
+                           : p == 3 ? &g.interval : &g.refill_min;
                 if (*field == vals[i]) continue;
                 *field = vals[i];
                 double ms;

@@ -434,9 +434,10 @@ __device__ __forceinline__ void ray_step(const TraceParams& P, RayState& R) {
         double cI = R.trans * dI;
         int ord = (int)(R.psi * (1.0 / PI));
         R.acc0 += cI;
-        if (ord == 1) R.acc1 += cI;
-        else if (ord == 2) R.acc2 += cI;
-        else if (ord >= 3) R.acc3 += cI;
+        // value selects, not a branch chain: the chain becomes a dynamically indexed stack array
+        R.acc1 += ord == 1 ? cI : 0.0;
+        R.acc2 += ord == 2 ? cI : 0.0;
+        R.acc3 += ord >= 3 ? cI : 0.0;
         R.trans *= bhm::exp(-dtau);
         if (R.trans < 1e-5) {
             R.fate = -1;

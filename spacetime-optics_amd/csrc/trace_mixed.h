@@ -407,9 +407,10 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             double cI = R.trans * dI;
             int ord = (int)(R.psi * 0.31830988618379067f);
             R.acc0 += cI;
-            if (ord == 1) R.acc1 += cI;
-            else if (ord == 2) R.acc2 += cI;
-            else if (ord >= 3) R.acc3 += cI;
+            // value selects, not a branch chain: the chain becomes a dynamically indexed stack array
+            R.acc1 += ord == 1 ? cI : 0.0;
+            R.acc2 += ord == 2 ? cI : 0.0;
+            R.acc3 += ord >= 3 ? cI : 0.0;
             R.trans *= (double)expf_(-(float)dtau);
             if (R.trans < 1e-5) {
                 R.fate = -1;
