@@ -26,7 +26,10 @@ using bhd::RayResult;
 using bhd::State;
 using bhd::TraceParams;
 
-#pragma clang fp contract(fast)
+// Multiply-adds fuse within one source expression only (not across statements): which operations
+// fuse is then fixed by the source, not by the inlining context, so a ray gives the same bits in
+// every kernel that runs it (frame kernel, persistent kernel).
+#pragma clang fp contract(on)
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpd(double x) {

@@ -93,8 +93,8 @@ def test_batch_turbulence_time_slices():
     """Frames at different times read different turbulence slices (linear in time between slices)."""
     cfg = setup(shot("c2_radio_1024.ini"), W=96, H=96)
     grid = bh.turbulence(64, 4, 60.0, 0.3, 5.0, 30.0, 10.0, 3)
-    t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], grid=grid, turb_half=60.0, turb_t0=0.0,
-                  turb_dt=10.0)
+    t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], grid=grid, turb_half=60.0,
+                  turb_t0=-1060.0, turb_dt=10.0)   # the flow is crossed ~940-1070 M before the frame time
     ini = shot("c2_radio_1024.ini")
     frames = [(cfg["S"], frame_for(cfg["S"], ini, 96, 96, t=tt)) for tt in (0.0, 4.0, 15.0, 29.0)]
     o = t.opts(precision=bh.PREC_FP64)

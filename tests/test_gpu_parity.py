@@ -133,16 +133,18 @@ def test_radio_frame_flux_256(oracle_bhm):
 
 
 def test_radio_time_slices(oracle_bhm):
-    """Time-sliced turbulence (4 slices 10 M apart): a 64^2 tile at frame times inside, between
-    and beyond the slices, per-pixel equal to the oracle."""
+    """Time-sliced turbulence (4 slices 10 M apart): a 64^2 tile at frame times whose flow
+    crossings (about 940-1070 M before the frame time) fall before, between and beyond the slices,
+    per-pixel equal to the oracle."""
     cfg = setup(shot("c2_radio_1024.ini"), W=256, H=256)
     grid = bh.turbulence(64, 4, 60.0, 0.3, 5.0, 30.0, 10.0, 3)
     t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], grid=grid, turb_half=60.0,
-                  turb_t0=-20.0, turb_dt=10.0)
-    OS = make_scene(scene_dict(cfg["S"]), k2=cfg["k2"], grid=grid, turb_half=60.0, turb_t0=-20.0, turb_dt=10.0)
+                  turb_t0=-1060.0, turb_dt=10.0)
+    OS = make_scene(scene_dict(cfg["S"]), k2=cfg["k2"], grid=grid, turb_half=60.0, turb_t0=-1060.0, turb_dt=10.0)
     OS.k2_l0 = cfg["l0"]
     OS.k2_dl = cfg["dl"]
     px, py = pixel_grid(64, 64, 96, 96)
+    seen = []
     for tf in (-35.0, 0.0, 3.7, 50.0):
         F = bh.Frame.from_buffer_copy(cfg["F"])
         F.t = tf
@@ -151,7 +153,9 @@ def test_radio_time_slices(oracle_bhm):
         gpu = t.trace_tile(F, 96, 96, 64, 64)
         _, ref, _ = oracle_bhm.trace_pixels(OS, OF, px, py, spp=1, jitter=False)
         assert_rays_equal(gpu, ref, f"t = {tf}")
+        seen.append(gpu["I"][..., 0].copy())
     OF.t = 0.0
+    assert not np.array_equal(seen[1], seen[2])   # 3.7 M apart, inside the slices: the field moved
 
 
 def test_radio_flux_jittered(oracle_bhm):
