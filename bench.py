@@ -72,10 +72,10 @@ def summarise(kernel_ms, wall, steps, rays, stats, prec):
     }
 
 
-def bench_mode(tracer, F, S, opts, steps, warmup, persistent):
-    """Times `steps` full-frame traces; persistent: one-frame bh_trace_batch (graph replay),
-    else bh_render_resident. Both return after the frame is done."""
-    run = ((lambda: tracer.trace_batch([(S, F)], opts, resident=True)) if persistent
+def bench_mode(tracer, F, S, opts, steps, warmup, persistent, cfg=None):
+    """Times `steps` full-frame traces; persistent: one-frame bh_trace_batch (graph replay, launch
+    configuration cfg), else bh_render_resident. Both return after the frame is done."""
+    run = ((lambda: tracer.trace_batch([(S, F)], opts, cfg=cfg, resident=True)) if persistent
            else (lambda: tracer.render_resident(F, opts)))
     for _ in range(warmup):
         run()
@@ -87,6 +87,24 @@ def bench_mode(tracer, F, S, opts, steps, warmup, persistent):
         stats = st
     wall = time.perf_counter() - t0
     return summarise(kernel_ms, wall, steps, F.W * F.H, stats, opts.precision)
+
+
+def calib_copy(mib):
+    """One device-to-device copy of `mib` MiB (the runtime's copy kernel): a transfer of known size
+    that scales the HBM counters of a profiling run (FETCH_SIZE / WRITE_SIZE)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    ok = ok and hip.hipMemset(a, 1, C.c_size_t(n)) == 0 and hip.hipDeviceSynchronize() == 0
+    ok = ok and hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0 and hip.hipDeviceSynchronize() == 0
+    for p in (a, b):
+        if p.value:
+            hip.hipFree(p)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    log(f"bench: calibration copy of {mib} MiB device to device")
 
 
 def frame_error(img, ref):
@@ -152,7 +170,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pixels", type=int, default=1048576)
     ap.add_argument("--modes", default=",".join(MODES))
-    ap.add_argument("--autotune", action="store_true", help="autotune the persistent launch before timing it")
+    ap.add_argument("--no-autotune", action="store_true", help="time the persistent modes with the default launch")
+    ap.add_argument("--no-mlp-train", action="store_true", help="skip MLP modes")
+    ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a device copy before the modes (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (rays shard with no "
@@ -174,6 +194,8 @@ def main():
     log(f"bench: n0 = {S.n0:.6g} cm^-3, flux {jy:.4f} Jy")
 
     setup = {}
+    if args.no_mlp_train:
+        modes = [m for m in modes if MODES[m][1] != bh.EMIS_MLP]
     if any(MODES[m][1] == bh.EMIS_MLP for m in modes):
         t0 = time.perf_counter()
         rep = tracer.mlp_train()
@@ -187,19 +209,22 @@ def main():
         setup["mlp_hot_cells"] = bake["hot_cells"]
         setup["mlp_failed_hot_cells"] = bake["failed_hot_cells"]
         log(f"bench: surrogate trained in {t1 - t0:.2f} s, baked in {t2 - t1:.2f} s: {setup}")
-    if args.autotune and any(MODES[m][2] for m in modes):
-        o = bh.Tracer.opts(precision=bh.PREC_MIXED)
-        best, ms, tried = tracer.autotune([(S, F)], o)
-        setup["autotuned"] = best.as_dict()
-        setup["autotune_configs"] = len(tried)
-        log(f"bench: autotuned persistent launch {best.as_dict()} ({ms:.2f} ms, {len(tried)} configs)")
+    if args.copy_calib > 0:
+        calib_copy(args.copy_calib)
 
     results = {}
     for m in modes:
         prec, emis, persistent = MODES[m]
         o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1)
         try:
-            results[m] = bench_mode(tracer, F, S, o, args.steps, args.warmup, persistent)
+            cfg = None
+            if persistent and not args.no_autotune:
+                t0 = time.perf_counter()
+                cfg, ms, tried = tracer.autotune([(S, F)], o)
+                setup[f"autotune_{m}"] = {"best": cfg.as_dict(), "ms": round(ms, 3), "configs": len(tried),
+                                          "s": round(time.perf_counter() - t0, 2)}
+                log(f"bench: {m}: autotuned {cfg.as_dict()} ({ms:.2f} ms, {len(tried)} configs)")
+            results[m] = bench_mode(tracer, F, S, o, args.steps, args.warmup, persistent, cfg)
             log(f"bench: {m}: {results[m]['mrays_per_s']:.3f} Mrays/s, kernel {results[m]['kernel_ms']:.2f} ms, "
                 f"{results[m]['achieved_tflops']:.2f} TFLOP/s ({100 * results[m]['roof_frac']:.1f}% of roof)")
         except bh.BhError as e:
@@ -210,7 +235,10 @@ def main():
     for m, r in results.items():
         prec, emis, persistent = MODES[m]
         o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1)
-        img = tracer.trace_batch([(S, F)], o)[0][0] if persistent else tracer.render(F, o)[0]
+        cfg = None
+        if f"autotune_{m}" in setup:
+            cfg = bh.BatchCfg(**setup[f"autotune_{m}"]["best"])
+        img = tracer.trace_batch([(S, F)], o, cfg=cfg)[0][0] if persistent else tracer.render(F, o)[0]
         r["error_vs_fp64"] = frame_error(img, ref)
         r["passes_gate"] = bool(r["error_vs_fp64"]["flux_rel"] < FLUX_GATE and
                                 r["error_vs_fp64"]["pixel_median_rel"] < PIXEL_GATE)
@@ -262,6 +290,11 @@ def main():
     }
     if fp64:
         line["speedup_vs_fp64"] = round(head["mrays_per_s"] / fp64["mrays_per_s"], 3)
+    fp64_modes = [m for m in results if MODES[m][0] == bh.PREC_FP64]
+    if fp64_modes:
+        best64 = max(fp64_modes, key=lambda m: results[m]["mrays_per_s"])
+        line["speedup_vs_best_fp64"] = {"mode": best64,
+                                        "x": round(head["mrays_per_s"] / results[best64]["mrays_per_s"], 3)}
     if cpu:
         line["gpu_vs_cpu"] = round(head["mrays_per_s"] / cpu["value"], 1)
     tracer.close()

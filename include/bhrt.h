@@ -168,12 +168,13 @@ typedef struct bh_batch_frame {
 
 typedef struct bh_batch_cfg {
     int block;          /* threads per workgroup: 64, 128 or 256 (0 -> 256) */
-    int grid_per_cu;    /* resident workgroups per compute unit (0 -> 2) */
-    int interval;       /* integration steps between retire/refill (0 -> 8) */
-    int refill_min;     /* free lanes needed to refill (0 -> 16) */
+    int grid_per_cu;    /* resident workgroups per compute unit (0 -> 4) */
+    int interval;       /* integration steps between retire/refill (0 -> 32) */
+    int refill_min;     /* free lanes needed to refill (0 -> 32) */
     int use_graph;      /* capture and replay the launch as a hipGraph */
     int waves;          /* 0: kernel compiled for the compiler's register budget; 3 or 4: compiled for at
-                           least that many resident waves per SIMD (fewer registers) */
+                           least that many resident waves per SIMD (fewer registers). A NULL cfg
+                           (and no autotuned one) means 256 / 4 / 32 / 32 / graph / 3. */
 } bh_batch_cfg;
 
 /* Traces every frame of `frames`; px_out (nullable) receives the frames one after the other,
@@ -194,7 +195,7 @@ int bh_autotune(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_trace
 typedef struct bh_mlp_train_cfg {
     int n_samples;      /* training samples (0 -> 2^22) */
     int batch;          /* samples per Adam step, multiple of 16 (0 -> 8192) */
-    int epochs;         /* passes over the samples (0 -> 40) */
+    int epochs;         /* passes over the samples (0 -> 200) */
     float lr, lr_min;   /* cosine-decayed Adam learning rate (0 -> 2e-3, 2e-5) */
     float hot_weight;   /* extra loss weight where ln f_j > -12 (< 0 -> 20) */
     uint64_t seed;      /* sampler and initialisation seed (0 -> 1) */
@@ -215,7 +216,10 @@ typedef struct bh_mlp_bake_report {
 /* Samples the exact routine over the box and trains the surrogate on the context's device. */
 int bh_mlp_train(bh_ctx* ctx, const bh_mlp_train_cfg* cfg, bh_mlp_report* report);
 /* Parameters (8706 floats: W1[3][64] b1 W2[64][64] b2 W3[64][64] b3 W4[64][2] b4) and the
-   normalisation (6 floats: log10-Theta centre, half-width, output means[2], scales[2]). */
+   normalisation (10 floats: input centres[3], input half-widths[3], output means[2], scales[2]).
+   The network's inputs are the surrogate coordinates s = (ln X, ln Theta_e, ln sin theta_B),
+   X = 10^u0 / ((2/9) Theta_e^2 sin theta_B), ln X clamped to at most 14, sin theta_B to at least
+   1e-3; network input k = (s_k - centre_k) / half-width_k. */
 int bh_mlp_get(bh_ctx* ctx, float* params, float* norm);
 int bh_mlp_set(bh_ctx* ctx, const float* params, const float* norm);
 /* Inference kernel on n raw inputs (log10 nu/nu_c, Theta_e, theta_B) -> (ln f_j, ln f_a). */
@@ -226,10 +230,11 @@ int bh_mlp_exact(bh_ctx* ctx, const float* in, int n, double* out);
    xn[n][3], raw targets tn[n][2] and weights w[n] (n a multiple of 16). */
 int bh_mlp_grad(bh_ctx* ctx, const float* params, const float* norm, const float* xn, const float* tn, const float* w,
                 int n, float* loss, float* grad);
-/* Tabulates the trained network on an n0 x n1 x n2 lattice over (log10 nu/nu_c, log10 Theta,
-   theta_B in [0, pi/2]) with the inference kernel and judges every cell against the exact routine
-   (|d ln f| > judge_tol fails it). The mixed tracer with BH_EMIS_MLP reads this table and uses the
-   exact formula in failed cells and outside the box. */
+/* Tabulates the trained network on an n0 x n1 x n2 lattice over the surrogate coordinates
+   (ln X in [-17, 14], ln Theta in [ln 0.5, ln 100], ln sin theta_B in [ln 1e-3, 0]; 0 -> 1024 x 128
+   x 8) with the inference kernel and judges every cell against the exact routine at four interior
+   points (|d ln f| > judge_tol, 0 -> 2e-3, fails it). The mixed tracer with BH_EMIS_MLP reads this
+   table and uses the exact formula in failed cells and outside the box. */
 int bh_mlp_bake(bh_ctx* ctx, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_report* report);
 
 /* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context

@@ -26,6 +26,8 @@ assert NPARAM == 8706
 LN_FJ_CONST = np.log(np.sqrt(2.0) * np.pi / 3.0 * 2.0 / 9.0)
 FLOOR_J, FLOOR_A = -40.0, -70.0
 TH_LO, TH_HI = 0.5, 100.0
+LX_LO, LX_HI = -17.0, 14.0       # ln X box of the surrogate coordinates
+LS_LO = np.log(1e-3)             # ln sin theta_B >= ln 1e-3
 
 
 def unpack(p):
@@ -59,8 +61,8 @@ def forward(p, xn):
 
 def loss_and_grad(p, norm, xn, t_raw, w):
     """Weighted MSE on normalised outputs (loss = sum_i w_i |y_i - t_i|^2 / n) and its gradient."""
-    mu = np.array([norm[2], norm[3]], dtype=np.float64)
-    sd = np.array([norm[4], norm[5]], dtype=np.float64)
+    mu = np.array([norm[6], norm[7]], dtype=np.float64)
+    sd = np.array([norm[8], norm[9]], dtype=np.float64)
     t = (np.asarray(t_raw, dtype=np.float64) - mu) / sd
     w = np.asarray(w, dtype=np.float64)
     n = len(w)
@@ -84,11 +86,29 @@ def loss_and_grad(p, norm, xn, t_raw, w):
     return loss, g
 
 
-def normalise_inputs(u0, theta, theta_b):
-    c1 = 0.5 * (np.log10(TH_LO) + np.log10(TH_HI))
-    h1 = 0.5 * (np.log10(TH_HI) - np.log10(TH_LO))
-    return np.column_stack([(np.asarray(u0) - 1.0) / 5.0, (np.log10(theta) - c1) / h1,
-                            np.asarray(theta_b) / (np.pi / 2) - 1.0])
+def surrogate_coords(u0, theta, theta_b):
+    """(ln X clamped to LX_HI, ln Theta, ln sin theta_B with sin >= 1e-3), X = 10^u0 / ((2/9) Theta^2 sin)."""
+    lth = np.log(np.asarray(theta, dtype=np.float64))
+    ls = np.log(np.maximum(np.sin(np.asarray(theta_b, dtype=np.float64)), 1e-3))
+    lx = np.asarray(u0, dtype=np.float64) * np.log(10.0) - np.log(2.0 / 9.0) - 2.0 * lth - ls
+    return np.minimum(lx, LX_HI), lth, ls
+
+
+def default_input_norm():
+    """Input centres[3] and half-widths[3] of the surrogate coordinates' box."""
+    lo = np.array([LX_LO, np.log(TH_LO), LS_LO])
+    hi = np.array([LX_HI, np.log(TH_HI), 0.0])
+    return 0.5 * (lo + hi), 0.5 * (hi - lo)
+
+
+def normalise_inputs(u0, theta, theta_b, norm=None):
+    """Network inputs; norm: the 10-float normalisation (default: the box's)."""
+    if norm is None:
+        c, h = default_input_norm()
+    else:
+        c, h = np.asarray(norm[0:3], dtype=np.float64), np.asarray(norm[3:6], dtype=np.float64)
+    s = np.column_stack(surrogate_coords(u0, theta, theta_b))
+    return (s - c) / h
 
 
 def table_lerp_f32(tab, x0, dx, x):
@@ -106,13 +126,20 @@ def table_lerp_f32(tab, x0, dx, x):
 def exact_lnf(u0, theta, theta_b, k2, k2_l0, k2_dl):
     u0 = np.asarray(u0, dtype=np.float64)
     lth = np.log(np.asarray(theta, dtype=np.float64))
-    ls = np.log(np.maximum(np.sin(np.asarray(theta_b, dtype=np.float64)), 1e-300))
+    ls = np.log(np.maximum(np.sin(np.asarray(theta_b, dtype=np.float64)), 1e-3))
     lnX = u0 * np.log(10.0) - np.log(2.0 / 9.0) - 2.0 * lth - ls
+    return exact_lnf_s(lnX, lth, ls, k2, k2_l0, k2_dl)
+
+
+def exact_lnf_s(lnX, lth, ls, k2, k2_l0, k2_dl):
+    """The same in the surrogate coordinates (ln X, ln Theta, ln sin theta_B)."""
+    lnX, lth, ls = (np.asarray(v, dtype=np.float64) for v in (lnX, lth, ls))
     X, x3 = np.exp(lnX), np.exp(lnX / 3.0)
     brk = np.sqrt(X) + 1.887748625363387 * np.sqrt(x3)
     lk2 = table_lerp_f32(k2, k2_l0, k2_dl, lth.astype(np.float32)).astype(np.float64)
     fj = LN_FJ_CONST + 2.0 * lth + ls - lk2 + 2.0 * np.log(brk) - x3
-    fa = fj - np.log(2.0) - lth - 2.0 * u0 * np.log(10.0)
+    u0ln10 = lnX + np.log(2.0 / 9.0) + 2.0 * lth + ls
+    fa = fj - np.log(2.0) - lth - 2.0 * u0ln10
     return fj, fa
 
 

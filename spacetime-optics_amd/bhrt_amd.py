@@ -26,6 +26,7 @@ EXPORTS = [
     "bh_trace_batch", "bh_autotune",
 ]
 MLP_NPARAM = 8706
+MLP_NNORM = 10   # input centres[3], half-widths[3], output means[2], scales[2]
 
 
 class Scene(C.Structure):
@@ -481,7 +482,7 @@ class Tracer:
 
     def mlp_get(self):
         p = np.zeros(MLP_NPARAM, dtype=np.float32)
-        nrm = np.zeros(6, dtype=np.float32)
+        nrm = np.zeros(MLP_NNORM, dtype=np.float32)
         self._check(self.L.bh_mlp_get(self.ctx, p.ctypes.data_as(_F), nrm.ctypes.data_as(_F)), "bh_mlp_get")
         return p, nrm
 

@@ -457,13 +457,15 @@ PersistFn pick_persist(int prec, bool radio, int waves) {
 
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
 
+// Defaults: the autotuner's choice on the 1024^2 230 GHz frame for both precisions (MI355X).
 bh_batch_cfg batch_defaults() {
     bh_batch_cfg d{};
     d.block = 256;
-    d.grid_per_cu = 2;
-    d.interval = 8;
-    d.refill_min = 16;
+    d.grid_per_cu = 4;
+    d.interval = 32;
+    d.refill_min = 32;
     d.use_graph = 1;
+    d.waves = 3;
     return d;
 }
 

@@ -5,8 +5,13 @@
    W[k][j] multiplying input k into output j.
 
    Inputs (the training box of the surrogate):
-     u0 = log10(nu / nu_c) in [-4, 6],  Theta_e in [0.5, 100] (log-uniform),  theta_B in (0, pi);
-   normalised to x = ((u0 - 1) / 5, (log10 Theta - c1) / h1, theta_B / (pi/2) - 1).
+     u0 = log10(nu / nu_c) in [-4, 6],  Theta_e in [0.5, 100] (log-uniform),  theta_B in (0, pi).
+   The network sees them through fixed coordinates in which the coefficients are close to
+   separable, s = (ln X, ln Theta_e, ln sin theta_B) with X = nu / nu_s = 10^u0 / ((2/9) Theta^2
+   sin theta_B), each mapped linearly onto [-1, 1] over its range:
+     ln X in [LX_LO, LX_HI] (clamped: above LX_HI both targets sit at their floors),
+     ln Theta in [ln 0.5, ln 100],  ln sin theta_B in [ln 1e-3, 0] (sin theta_B >= 1e-3 as in the
+     tracer's flow model).
    Outputs (normalised by per-output mean and scale):
      y0 ~ ln f_j,  f_j = j / (n_e e^2 nu_c / c)
      y1 ~ ln f_a,  f_a = alpha / (n_e e^2 / (m_e c nu_c)) = f_j / (2 Theta (nu/nu_c)^2)
@@ -29,19 +34,27 @@ constexpr double TB_LO = 0.0, TB_HI = 3.14159265358979323846;
 constexpr double FLOOR_J = -40.0, FLOOR_A = -70.0;
 // ln(sqrt(2) pi / 3 * 2 / 9)
 constexpr double LN_FJ_CONST = -1.1113862093150109;
+// surrogate coordinates s = (ln X, ln Theta, ln sin theta_B): box
+constexpr double LX_LO = -17.0, LX_HI = 14.0;
+constexpr double LTH_LO = -0.6931471805599453, LTH_HI = 4.605170185988092;   // ln 0.5, ln 100
+constexpr double LS_LO = -6.907755278982137, LS_HI = 0.0;                    // ln 1e-3, ln 1
+constexpr double LN_2_9 = -1.5040773967762742;                              // ln(2/9)
+constexpr double LN10 = 2.302585092994046;
 
 struct Norm {
-    float x_c1, x_h1;          // log10 Theta centre and half-width
+    float x_c[3], x_h[3];      // input centre and half-width per surrogate coordinate
     float y_mu[2], y_sd[2];    // output normalisation
 };
+constexpr int NORM_FLOATS = 10;   // x_c[3], x_h[3], y_mu[2], y_sd[2] (bh_mlp_get / bh_mlp_set order)
 
-// Surrogate deployment: the trained network tabulated over its box, plus per-cell trust flags set by
-// the exact routine (flagged cells fall back to the exact formula in the tracer).
+// Surrogate deployment: the trained network tabulated over its box in the surrogate coordinates,
+// plus per-cell trust flags set by the exact routine (flagged cells fall back to the exact formula
+// in the tracer).
 struct SurrogateTable {
-    const float* tab;      // interleaved pairs     // [n2][n1][n0][2] (ln f_j, ln f_a)
+    const float* tab;           // [n2][n1][n0] pairs (ln f_j, ln f_a)
     const unsigned char* bad;   // [(n2-1)][(n1-1)][(n0-1)] cells that failed the exact-routine judge
     int n0, n1, n2;
-    float lo0, lo1, lo2, inv_d0, inv_d1, inv_d2;
+    float lo0, lo1, lo2, inv_d0, inv_d1, inv_d2, hi0;
 };
 
 }  // namespace bhmlp

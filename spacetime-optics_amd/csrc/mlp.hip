@@ -200,23 +200,27 @@ __device__ __forceinline__ double uniform01(uint64_t seed, uint64_t i) {   // (0
 
 }  // namespace
 
-// Exact coefficients of the surrogate's box (emission.cuh:159-170 normalised; FP64, bh_math
-// elementary functions, the tracer's K2 table): returns (ln f_j, ln f_a) for
-// u0 = log10(nu/nu_c), Theta, theta_B.
-__device__ void exact_lnf(const float* __restrict__ k2, int k2_n, float k2_l0, float k2_dl, double u0, double th,
-                          double tb, double* lfj, double* lfa) {
-    constexpr double LN10 = 2.302585092994046, LN2_9 = -1.5040773967762742;
-    double s, c;
-    bhm::sincos(tb, &s, &c);
-    s = fmax(s, 1e-300);
-    const double lth = bhm::log(th), ls = bhm::log(s);
-    const double lnX = u0 * LN10 - LN2_9 - 2.0 * lth - ls;
-    const double X = bhm::exp(lnX), x3 = bhm::exp(lnX * (1.0 / 3.0));
+// Exact coefficients in the surrogate coordinates (emission.cuh:159-170 normalised; FP64, bh_math
+// elementary functions, the tracer's K2 table): (ln f_j, ln f_a) at ln X = lx, ln Theta = lth,
+// ln sin theta_B = ls.
+__device__ void exact_lnf_s(const float* __restrict__ k2, int k2_n, float k2_l0, float k2_dl, double lx, double lth,
+                            double ls, double* lfj, double* lfa) {
+    const double X = bhm::exp(lx), x3 = bhm::exp(lx * (1.0 / 3.0));
     const double brk = sqrt(X) + 1.887748625363387 * sqrt(x3);
     const double lk2 = (double)bhd::table_lerp(k2, k2_n, k2_l0, k2_dl, (float)lth);
     const double fj = LN_FJ_CONST + 2.0 * lth + ls - lk2 + 2.0 * bhm::log(brk) - x3;
+    const double u0ln10 = lx + LN_2_9 + 2.0 * lth + ls;
     *lfj = fj;
-    *lfa = fj - 0.6931471805599453 - lth - 2.0 * u0 * LN10;
+    *lfa = fj - 0.6931471805599453 - lth - 2.0 * u0ln10;
+}
+
+// The same at u0 = log10(nu/nu_c), Theta, theta_B (sin theta_B >= 1e-3, as in the tracer).
+__device__ void exact_lnf(const float* __restrict__ k2, int k2_n, float k2_l0, float k2_dl, double u0, double th,
+                          double tb, double* lfj, double* lfa) {
+    double s, c;
+    bhm::sincos(tb, &s, &c);
+    const double lth = bhm::log(th), ls = bhm::log(fmax(s, 1e-3));
+    exact_lnf_s(k2, k2_n, k2_l0, k2_dl, u0 * LN10 - LN_2_9 - 2.0 * lth - ls, lth, ls, lfj, lfa);
 }
 
 namespace {
@@ -229,13 +233,16 @@ __global__ __launch_bounds__(256) void k_mlp_sample(uint64_t seed, int n, const 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double u0 = U0_LO + (U0_HI - U0_LO) * uniform01(seed, 3ull * i);
-    const double lth = bhm::log(TH_LO) + (bhm::log(TH_HI) - bhm::log(TH_LO)) * uniform01(seed, 3ull * i + 1);
+    const double lth = LTH_LO + (LTH_HI - LTH_LO) * uniform01(seed, 3ull * i + 1);
     const double tb = TB_LO + (TB_HI - TB_LO) * uniform01(seed, 3ull * i + 2);
+    double sb, cb;
+    bhm::sincos(tb, &sb, &cb);
+    const double ls = bhm::log(fmax(sb, 1e-3));
+    const double lx = u0 * LN10 - LN_2_9 - 2.0 * lth - ls;
     double fj, fa;
-    exact_lnf(k2, k2_n, k2_l0, k2_dl, u0, bhm::exp(lth), tb, &fj, &fa);
-    xs[3 * (size_t)i] = (float)((u0 - 1.0) / 5.0);
-    xs[3 * (size_t)i + 1] = (float)((lth * 0.4342944819032518 - N.x_c1) / N.x_h1);
-    xs[3 * (size_t)i + 2] = (float)(tb / 1.5707963267948966 - 1.0);
+    exact_lnf_s(k2, k2_n, k2_l0, k2_dl, lx, lth, ls, &fj, &fa);
+    const double sc[3] = {fmin(lx, LX_HI), lth, ls};
+    for (int k = 0; k < 3; ++k) xs[3 * (size_t)i + k] = (float)((sc[k] - N.x_c[k]) / N.x_h[k]);
     ts[2 * (size_t)i] = (float)fmax(fj, FLOOR_J);
     ts[2 * (size_t)i + 1] = (float)fmax(fa, FLOOR_A);
     ws[i] = 1.f + hot_weight / (1.f + expf(-(float)(fj + 12.0)));
@@ -392,22 +399,22 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ P, N
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         if (lane < TM) {
             const int s = tile * TM + lane;
-            float u0 = 0.f, lth10 = 0.f, tb = 0.f;
+            float sc[3] = {0.f, 0.f, 0.f};   // surrogate coordinates
             if (s < n) {
                 if (lattice) {
                     const int i0 = s % L.n0, i1 = (s / L.n0) % L.n1, i2 = s / (L.n0 * L.n1);
-                    u0 = L.lo0 + i0 * L.d0;
-                    lth10 = L.lo1 + i1 * L.d1;
-                    tb = L.lo2 + i2 * L.d2;
+                    sc[0] = L.lo0 + i0 * L.d0;
+                    sc[1] = L.lo1 + i1 * L.d1;
+                    sc[2] = L.lo2 + i2 * L.d2;
                 } else {
-                    u0 = in[3 * (size_t)s];
-                    lth10 = log10f(in[3 * (size_t)s + 1]);
-                    tb = in[3 * (size_t)s + 2];
+                    const float u0 = in[3 * (size_t)s], lth = logf(in[3 * (size_t)s + 1]);
+                    const float ls = logf(fmaxf(sinf(in[3 * (size_t)s + 2]), 1e-3f));
+                    sc[0] = fminf(u0 * (float)LN10 - (float)LN_2_9 - 2.f * lth - ls, (float)LX_HI);
+                    sc[1] = lth;
+                    sc[2] = ls;
                 }
             }
-            T.x[lane][0] = (u0 - 1.f) / 5.f;
-            T.x[lane][1] = (lth10 - N.x_c1) / N.x_h1;
-            T.x[lane][2] = tb / 1.5707963267948966f - 1.f;
+            for (int k = 0; k < 3; ++k) T.x[lane][k] = (sc[k] - N.x_c[k]) / N.x_h[k];
         }
         __syncthreads();
         forward_tile(P, W, T);
@@ -455,9 +462,9 @@ __global__ __launch_bounds__(256) void k_mlp_judge(const float2* __restrict__ ta
             float c0v = c00 + (c10 - c00) * f1, c1v = c01 + (c11 - c01) * f1;
             iv[o] = c0v + (c1v - c0v) * f2;
         }
-        const double u0 = L.lo0 + (i0 + f0) * L.d0, lth10 = L.lo1 + (i1 + f1) * L.d1, tb = L.lo2 + (i2 + f2) * L.d2;
+        const double lx = L.lo0 + (i0 + f0) * L.d0, lth = L.lo1 + (i1 + f1) * L.d1, ls = L.lo2 + (i2 + f2) * L.d2;
         double fj, fa;
-        exact_lnf(k2, k2_n, k2_l0, k2_dl, u0, bhm::exp(lth10 * 2.302585092994046), tb, &fj, &fa);
+        exact_lnf_s(k2, k2_n, k2_l0, k2_dl, lx, lth, ls, &fj, &fa);
         if (fj > floor_j) {
             hot = true;
             if (fabs(iv[0] - fj) > tol) fail = true;
@@ -484,10 +491,37 @@ namespace {
 
 Norm default_norm() {
     Norm N;
-    N.x_c1 = (float)(0.5 * (std::log10(TH_LO) + std::log10(TH_HI)));
-    N.x_h1 = (float)(0.5 * (std::log10(TH_HI) - std::log10(TH_LO)));
+    const double lo[3] = {LX_LO, LTH_LO, LS_LO}, hi[3] = {LX_HI, LTH_HI, LS_HI};
+    for (int k = 0; k < 3; ++k) {
+        N.x_c[k] = (float)(0.5 * (lo[k] + hi[k]));
+        N.x_h[k] = (float)(0.5 * (hi[k] - lo[k]));
+    }
     N.y_mu[0] = N.y_mu[1] = 0.f;
     N.y_sd[0] = N.y_sd[1] = 1.f;
+    return N;
+}
+
+void norm_to_floats(const Norm& N, float* v) {
+    for (int k = 0; k < 3; ++k) {
+        v[k] = N.x_c[k];
+        v[3 + k] = N.x_h[k];
+    }
+    v[6] = N.y_mu[0];
+    v[7] = N.y_mu[1];
+    v[8] = N.y_sd[0];
+    v[9] = N.y_sd[1];
+}
+
+Norm norm_from_floats(const float* v) {
+    Norm N;
+    for (int k = 0; k < 3; ++k) {
+        N.x_c[k] = v[k];
+        N.x_h[k] = v[3 + k];
+    }
+    N.y_mu[0] = v[6];
+    N.y_mu[1] = v[7];
+    N.y_sd[0] = v[8];
+    N.y_sd[1] = v[9];
     return N;
 }
 
@@ -537,7 +571,7 @@ int bh_mlp_train(bh_ctx* c, const bh_mlp_train_cfg* cfg_in, bh_mlp_report* rep) 
     if (cfg_in) cfg = *cfg_in;
     if (cfg.n_samples <= 0) cfg.n_samples = 1 << 22;
     if (cfg.batch <= 0) cfg.batch = 8192;
-    if (cfg.epochs <= 0) cfg.epochs = 40;
+    if (cfg.epochs <= 0) cfg.epochs = 200;
     if (!(cfg.lr > 0)) cfg.lr = 2e-3f;
     if (!(cfg.lr_min > 0)) cfg.lr_min = 2e-5f;
     if (cfg.seed == 0) cfg.seed = 1;
@@ -658,11 +692,7 @@ int bh_mlp_get(bh_ctx* c, float* params, float* norm) {
     }
     MCK(c, hipSetDevice(c->device));
     MCK(c, hipMemcpy(params, c->d_mlp, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
-    if (norm) {
-        float v[6] = {c->mlp_norm.x_c1, c->mlp_norm.x_h1, c->mlp_norm.y_mu[0], c->mlp_norm.y_mu[1], c->mlp_norm.y_sd[0],
-                      c->mlp_norm.y_sd[1]};
-        std::memcpy(norm, v, sizeof v);
-    }
+    if (norm) norm_to_floats(c->mlp_norm, norm);
     return BH_OK;
 }
 
@@ -672,12 +702,7 @@ int bh_mlp_set(bh_ctx* c, const float* params, const float* norm) {
     int rc = ensure_params(c);
     if (rc) return rc;
     MCK(c, hipMemcpy(c->d_mlp, params, sizeof(float) * NPARAM, hipMemcpyHostToDevice));
-    c->mlp_norm.x_c1 = norm[0];
-    c->mlp_norm.x_h1 = norm[1];
-    c->mlp_norm.y_mu[0] = norm[2];
-    c->mlp_norm.y_mu[1] = norm[3];
-    c->mlp_norm.y_sd[0] = norm[4];
-    c->mlp_norm.y_sd[1] = norm[5];
+    c->mlp_norm = norm_from_floats(norm);
     c->mlp_trained = true;
     c->stab_ready = false;
     return BH_OK;
@@ -747,13 +772,7 @@ int bh_mlp_grad(bh_ctx* c, const float* params, const float* norm, const float* 
                 int n, float* loss_out, float* grad_out) {
     if (!c || !params || !norm || !xn || !tn || !w || !grad_out || n <= 0 || n % TM != 0) return BH_ERR_ARG;
     MCK(c, hipSetDevice(c->device));
-    Norm N = default_norm();
-    N.x_c1 = norm[0];
-    N.x_h1 = norm[1];
-    N.y_mu[0] = norm[2];
-    N.y_mu[1] = norm[3];
-    N.y_sd[0] = norm[4];
-    N.y_sd[1] = norm[5];
+    const Norm N = norm_from_floats(norm);
     const int tiles = 4, nwg = (n / TM + tiles - 1) / tiles;
     float *dp = nullptr, *dx = nullptr, *dt = nullptr, *dw = nullptr, *part = nullptr, *lp = nullptr, *g = nullptr,
           *l = nullptr;
@@ -791,9 +810,9 @@ int bh_mlp_bake(bh_ctx* c, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_
         c->err = "baking needs trained surrogate parameters and the K2 table";
         return BH_ERR_ARG;
     }
-    if (n0 <= 0) n0 = 160;
-    if (n1 <= 0) n1 = 64;
-    if (n2 <= 0) n2 = 32;
+    if (n0 <= 0) n0 = 1024;
+    if (n1 <= 0) n1 = 128;
+    if (n2 <= 0) n2 = 8;
     if (!(judge_tol > 0)) judge_tol = 2e-3f;
     if (n0 < 2 || n1 < 2 || n2 < 2 || (size_t)n0 * n1 * n2 > (size_t)1 << 24) {
         c->err = "bad surrogate lattice size";
@@ -810,17 +829,17 @@ int bh_mlp_bake(bh_ctx* c, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_
     MCK(c, hipMalloc((void**)&c->d_sbad, cells));
     unsigned int* counts = nullptr;
     MCK(c, hipMalloc((void**)&counts, 2 * sizeof(unsigned int)));
-    // the formula depends on sin(theta_B): the table covers theta_B in [0, pi/2]
+    // lattice over the surrogate coordinates (ln X, ln Theta, ln sin theta_B)
     Lattice L;
     L.n0 = n0;
     L.n1 = n1;
     L.n2 = n2;
-    L.lo0 = (float)U0_LO;
-    L.d0 = (float)((U0_HI - U0_LO) / (n0 - 1));
-    L.lo1 = (float)std::log10(TH_LO);
-    L.d1 = (float)((std::log10(TH_HI) - std::log10(TH_LO)) / (n1 - 1));
-    L.lo2 = 0.f;
-    L.d2 = (float)(1.5707963267948966 / (n2 - 1));
+    L.lo0 = (float)LX_LO;
+    L.d0 = (float)((LX_HI - LX_LO) / (n0 - 1));
+    L.lo1 = (float)LTH_LO;
+    L.d1 = (float)((LTH_HI - LTH_LO) / (n1 - 1));
+    L.lo2 = (float)LS_LO;
+    L.d2 = (float)((LS_HI - LS_LO) / (n2 - 1));
     double t0 = now_ms();
     const int ntiles = (int)((nodes + TM - 1) / TM);
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::min(ntiles, 8192)), dim3(64), 0, c->st, c->d_mlp, c->mlp_norm, nullptr,
@@ -861,6 +880,7 @@ int bh_mlp_bake(bh_ctx* c, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_
     T.inv_d0 = 1.f / L.d0;
     T.inv_d1 = 1.f / L.d1;
     T.inv_d2 = 1.f / L.d2;
+    T.hi0 = (float)LX_HI;
     c->stab_ready = true;
     if (rep) {
         rep->ms_bake = now_ms() - t0;

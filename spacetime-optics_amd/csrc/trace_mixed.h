@@ -274,9 +274,12 @@ __device__ __forceinline__ void transfer(const EmisParams& E, const FlowSample& 
 // Surrogate lookup: (ln f_j, ln f_a) of the tabulated network at (u0 = log10 nu/nu_c,
 // log10 Theta, theta_B folded into [0, pi/2]); false outside the box or in a cell the exact routine
 // judged inaccurate (the caller then evaluates the exact formula).
-__device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T, float u0, float lth10, float tb,
+// Trilinear lookup of the baked surrogate at surrogate coordinates (ln X, ln Theta, ln sin theta_B);
+// false outside the table or in a cell the exact-routine judge flagged. ln X above the table is
+// clamped to its edge, where both coefficients sit at their floors.
+__device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T, float lx, float lth, float ls,
                                                  float* fj, float* fa) {
-    const float g0 = (u0 - T.lo0) * T.inv_d0, g1 = (lth10 - T.lo1) * T.inv_d1, g2 = (tb - T.lo2) * T.inv_d2;
+    const float g0 = (fminf(lx, T.hi0) - T.lo0) * T.inv_d0, g1 = (lth - T.lo1) * T.inv_d1, g2 = (ls - T.lo2) * T.inv_d2;
     if (!(g0 >= 0.f && g1 >= 0.f && g2 >= 0.f && g0 <= (float)(T.n0 - 1) && g1 <= (float)(T.n1 - 1) &&
           g2 <= (float)(T.n2 - 1)))
         return false;
@@ -396,10 +399,9 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         FlowSample f;
         if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
             float ln_j, ln_a, fj, fa;
-            const float sinb = expf_(f.ln_sinb);
             if (M.use_surrogate &&
-                surrogate_lookup(M.S, (f.ln_nu - f.ln_nuc) * 0.43429448190325176f, f.ln_th * 0.43429448190325176f,
-                                 asinf(fminf(sinb, 1.f)), &fj, &fa)) {
+                surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
+                                 f.ln_sinb, &fj, &fa)) {
                 ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
                 ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
             } else {

@@ -24,7 +24,8 @@ def ctx():
 def random_params(seed, scale=0.25):
     rng = np.random.default_rng(seed)
     p = rng.normal(size=M.NPARAM) * scale
-    norm = np.array([0.8494850021680094, 1.1505149978319906, -6.0, -30.0, 8.0, 15.0], dtype=np.float32)
+    c, h = M.default_input_norm()
+    norm = np.concatenate([c, h, [-6.0, -30.0, 8.0, 15.0]]).astype(np.float32)
     return p.astype(np.float32), norm
 
 
@@ -37,9 +38,9 @@ def test_mfma_forward_matches_oracle(ctx):
     u0, th, tb = rng.uniform(-4, 6, n), np.exp(rng.uniform(np.log(0.5), np.log(100), n)), rng.uniform(0, np.pi, n)
     got = t.mlp_infer(u0, th, tb)
     x32 = np.column_stack([u0, th, tb]).astype(np.float32).astype(np.float64)
-    y, _ = M.forward(p, M.normalise_inputs(x32[:, 0], x32[:, 1], x32[:, 2]))
-    want = y * norm[4:6] + norm[2:4]
-    err = np.abs(got - want) / norm[4:6]
+    y, _ = M.forward(p, M.normalise_inputs(x32[:, 0], x32[:, 1], x32[:, 2], norm))
+    want = y * norm[8:10] + norm[6:8]
+    err = np.abs(got - want) / norm[8:10]
     print(f"forward: max normalised error {err.max():.2e}")
     assert err.max() < 1e-4   # split-BF16 products keep ~16 significant bits
 
@@ -50,7 +51,7 @@ def test_mfma_gradient_matches_oracle(ctx):
     rng = np.random.default_rng(4)
     n = 256
     xn = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
-    tr = (rng.normal(size=(n, 2)) * norm[4:6] + norm[2:4]).astype(np.float32)
+    tr = (rng.normal(size=(n, 2)) * norm[8:10] + norm[6:8]).astype(np.float32)
     w = rng.uniform(0.5, 3.0, n).astype(np.float32)
     loss, g = t.mlp_grad(p, norm, xn, tr, w)
     loss_ref, g_ref = M.loss_and_grad(p.astype(np.float64), norm, xn, tr, w)
@@ -79,7 +80,7 @@ def test_exact_sampler_routine_matches_oracle(ctx):
 @pytest.fixture(scope="module")
 def trained(ctx):
     t, cfg = ctx
-    rep = t.mlp_train(n_samples=1 << 22, batch=8192, epochs=30, seed=1)
+    rep = t.mlp_train(seed=1)   # 2^22 samples, batch 8192, 200 epochs
     print(f"training: {rep}")
     return rep
 
@@ -94,16 +95,17 @@ def test_training_converges_and_is_accurate(ctx, trained):
     got = t.mlp_infer(u0, th, tb)
     ex = t.mlp_exact(u0, th, tb)
     dj = np.abs(got[:, 0] - ex[:, 0])
+    da = np.abs(got[:, 1] - ex[:, 1])
     print(f"surrogate |d ln f_j| in the emitting region: median {np.median(dj):.2e} p99 {np.percentile(dj, 99):.2e} "
-          f"max {dj.max():.2e}")
-    assert np.median(dj) < 1e-2
+          f"max {dj.max():.2e}; |d ln f_a| median {np.median(da):.2e} max {da.max():.2e}")
+    assert np.median(dj) < 2e-3
 
 
 def test_surrogate_in_tracer_and_gain(ctx, trained):
     """What the trained network gains: mixed tracer with the baked surrogate vs the exact FP32
     formula and vs the plain FP64 tracer on the 1024^2 frame."""
     t, cfg = ctx
-    bake = t.mlp_bake(160, 64, 32, 2e-3)
+    bake = t.mlp_bake()
     print(f"bake: {bake}")
     F = cfg["F"]
     ref, st64 = t.render(F, bh.Tracer.opts(precision=bh.PREC_FP64))
@@ -117,6 +119,8 @@ def test_surrogate_in_tracer_and_gain(ctx, trained):
         med = np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask])
         print(f"{name}: flux err {flux:.3e}, pixel median {med:.3e}, kernel {st.ms_kernel:.2f} ms "
               f"(fp64 {st64.ms_kernel:.2f} ms, speed-up {st64.ms_kernel / st.ms_kernel:.2f}x)")
+    print(f"surrogate vs exact emission inside the mixed tracer: {stx.ms_kernel / stm.ms_kernel:.3f}x")
     Ia = mlp[..., 0].astype(np.float64)
-    assert abs(Ia.sum() - Ib.sum()) / Ib.sum() < 1e-2
+    assert abs(Ia.sum() - Ib.sum()) / Ib.sum() < 1e-3
+    assert np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask]) < 1e-2
     assert stm.ms_kernel < st64.ms_kernel

@@ -17,7 +17,7 @@ def test_backward_matches_finite_differences():
     xn = rng.uniform(-1, 1, (16, 3))
     t = rng.normal(size=(16, 2))
     w = rng.uniform(0.5, 2.0, 16)
-    norm = np.array([0.85, 1.15, 0.3, -0.2, 2.0, 1.5])
+    norm = np.array([-1.5, 1.95, -3.45, 15.5, 2.65, 3.45, 0.3, -0.2, 2.0, 1.5])
     loss, g = M.loss_and_grad(p, norm, xn, t, w)
     idx = np.concatenate([rng.choice(np.arange(M.OFF_W1, M.OFF_B1), 8), rng.choice(np.arange(M.OFF_W2, M.OFF_B2), 12),
                           rng.choice(np.arange(M.OFF_W3, M.OFF_B3), 12), np.arange(M.OFF_W4, M.NPARAM),
