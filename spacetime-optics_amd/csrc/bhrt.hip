@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -339,16 +340,71 @@ KernelFn pick_kernel(int prec, bool radio, bool rays) {
     return rays ? k_trace<BH_PREC_MIXED, false, true> : k_trace<BH_PREC_MIXED, false, false>;
 }
 
+constexpr double C_C = 2.99792458e10, C_H = 6.62607015e-27, C_ME = 9.1093837e-28, C_MP = 1.67262192e-24,
+                 C_QE = 4.80320471e-10, C_PI = 3.14159265358979323846;
+
+/* Radius outside which the scene's flow model emits a fraction `frac` of its emission: the thermal
+   synchrotron emissivity (emission.cuh:130-170 at nu_obs, in the mid-plane, sin theta_B = 1, no
+   Doppler factor, no turbulence) weighted by r^2 (a face-on annulus of height ~ h_r r), integrated
+   outward from the horizon on a fine radial grid. Host-side, once per launch. */
+double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2_l0, float k2_dl, double frac) {
+    const int n = 2000;
+    const double r0 = S.rh * 1.001, r1 = S.r_flow_max;
+    if (!(r1 > r0) || k2.size() < 2) return r1;
+    std::vector<double> w(n);
+    double tot = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double r = r0 + (r1 - r0) * (i + 0.5) / n;
+        const double ne = S.n0 * std::pow(r, -S.n_index) * (1.0 - std::exp(-(r - S.rh) * (r - S.rh) / 0.25));
+        const double th = S.theta0 * std::pow(r, -S.theta_index);
+        const double B = std::sqrt(8.0 * C_PI * ne * C_MP * C_C * C_C / (6.0 * r * S.beta_p));
+        const double nus = 2.0 / 9.0 * (C_QE * B / (2.0 * C_PI * C_ME * C_C)) * th * th;
+        const double X = S.nu_obs / nus, x3 = std::cbrt(X);
+        float f = ((float)std::log(th) - k2_l0) / k2_dl;
+        f = std::min(std::max(f, 0.f), (float)(k2.size() - 2) + 0.9999f);
+        const int k = (int)f;
+        const double lk2 = k2[k] + (k2[k + 1] - k2[k]) * (f - k);
+        const double brk = std::sqrt(X) + 1.887748625363387 * std::sqrt(x3);
+        const double j = ne * nus * std::exp(-lk2) * brk * brk * std::exp(-x3);
+        w[i] = std::isfinite(j) ? j * r * r : 0.0;
+        tot += w[i];
+    }
+    if (!(tot > 0.0)) return r1;
+    double out = 0.0;
+    for (int i = n - 1; i >= 0; --i) {
+        out += w[i];
+        if (out > frac * tot) return r0 + (r1 - r0) * (i + 1.0) / n;
+    }
+    return r0;
+}
+
 // Log-space emission constants and the region split of the mixed policy
-bhm32::MixedParams make_mixed(const bh_scene& S, const bh_trace_opts& o) {
-    constexpr double C = 2.99792458e10, H = 6.62607015e-27, ME = 9.1093837e-28, MP = 1.67262192e-24,
-                     QE = 4.80320471e-10, PI = 3.14159265358979323846;
+bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_trace_opts& o) {
+    constexpr double C = C_C, H = C_H, ME = C_ME, MP = C_MP, QE = C_QE, PI = C_PI;
     bhm32::MixedParams M;
     std::memset(&M, 0, sizeof M);
     M.tol_far = o.tol_far > 0 ? o.tol_far : std::max(1e-6, S.tol);
     const double r_ph = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::min(1.0, std::fabs(S.a)))));
     const double r_near = o.r_near > 0 ? o.r_near : std::max(3.0 * S.rh, r_ph) + 1.0;
-    M.r_far = (S.mode == BH_MODE_RADIO && S.flow_on) ? std::max(S.r_flow_max, r_near) : r_near;
+    M.r_far = r_near;
+    if (S.mode == BH_MODE_RADIO && S.flow_on) {
+        double r_emit = o.r_emit;
+        if (!(r_emit > 0)) {
+            // cached per flow model: a batch of frames usually shares a few scenes
+            const double key[9] = {S.n0, S.n_index, S.theta0, S.theta_index, S.beta_p, S.nu_obs, S.rh, S.r_flow_max,
+                                   (double)c->k2_l0};
+            for (const auto& e : c->emit_cache)
+                if (std::memcmp(e.first.data(), key, sizeof key) == 0) r_emit = e.second;
+            if (!(r_emit > 0)) {
+                r_emit = emission_radius(S, c->k2_host, c->k2_l0, c->k2_dl, 1e-6);
+                if (c->emit_cache.size() >= 64) c->emit_cache.clear();
+                std::array<double, 9> k;
+                std::memcpy(k.data(), key, sizeof key);
+                c->emit_cache.emplace_back(k, r_emit);
+            }
+        }
+        M.r_far = std::max(std::min(r_emit, S.r_flow_max), r_near);
+    }
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);
     E.n_index = (float)S.n_index;
@@ -393,7 +449,7 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     if (opts) o = *opts;
     rc = check_opts(c, o);
     if (rc) return rc;
-    bhm32::MixedParams M = make_mixed(c->S, o);
+    bhm32::MixedParams M = make_mixed(c, c->S, o);
     if (o.emissivity == BH_EMIS_MLP) {
         M.use_surrogate = 1;
         M.S = c->stab;
@@ -507,7 +563,7 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
         B.P.h = f->H;
         B.P.spp = 1;
         B.P.jitter = o.jitter;
-        B.M = make_mixed(S, o);
+        B.M = make_mixed(c, S, o);
         if (o.emissivity == BH_EMIS_MLP) {
             B.M.use_surrogate = 1;
             B.M.S = c->stab;
@@ -655,6 +711,7 @@ int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const 
             c->k2_n = tables->k2_n;
             c->k2_l0 = tables->k2_l0;
             c->k2_dl = tables->k2_dl;
+            c->k2_host.assign(tables->k2_ln, tables->k2_ln + tables->k2_n);
             if (hipMalloc((void**)&c->d_k2, sizeof(float) * c->k2_n) != hipSuccess ||
                 hipMemcpy(c->d_k2, tables->k2_ln, sizeof(float) * c->k2_n, hipMemcpyHostToDevice) != hipSuccess) {
                 c->err = "K2 table upload failed";

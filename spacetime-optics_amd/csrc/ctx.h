@@ -2,7 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/bhrt.h"
 #include "mlp.h"
@@ -22,6 +25,8 @@ struct bh_ctx {
     bh_scene S{};
     bh_settings rs{};
     float* d_k2 = nullptr;
+    std::vector<float> k2_host;   // host copy of the K2 table (emission-radius estimate)
+    std::vector<std::pair<std::array<double, 9>, double>> emit_cache;   // flow model -> emission radius
     int k2_n = 0;
     float k2_l0 = 0, k2_dl = 0;
     float* d_turb = nullptr;
