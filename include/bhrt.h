@@ -159,6 +159,16 @@ int bh_radio_flux(bh_ctx* ctx, const bh_frame* frame, int n, double pix_sr, doub
 int bh_vacuum_fates(bh_ctx* ctx, const double* photons, int n, double tol, double r_escape, int max_steps,
                     int* fates);
 
+/* Renderer::render's adaptive sampling (render.cu:677-721, k_pass1 / k_adapt / k_dilate / scan /
+   k_pass2 / k_resolve, render.cu:89-242): settings.spp samples per pixel (opts->spp > 0 overrides),
+   then, where the standard error of the pixel's luminance exceeds settings.target_err of (its mean +
+   3% of the frame mean), more samples up to settings.spp_max, in chunks of 8, dilated over the 3 x 3
+   neighbourhood (defaults 8 / 128 / 0.03 when the settings are 0). Samples continue each pixel's
+   R2 + PCG sequence (jittered when opts->jitter). Sums are deterministic: a pixel is the FP64 sum of
+   its samples in sample order over its sample count. nsamp_out (nullable): samples per pixel. */
+int bh_render_adaptive(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, float* px_out,
+                       unsigned int* nsamp_out, bh_stats* stats);
+
 /* ---- Persistent batched tracing (new; the reference renders one frame per launch sequence,
    render.cu:677-730). Frames of different size and scene (e.g. spin) are traced by one persistent
    launch: resident workgroups pull 8 x 8 pixel tiles from per-frame work queues, retire finished

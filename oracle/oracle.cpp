@@ -12,6 +12,7 @@
    (the functions the GPU tracer uses). sqrt, division and fmin/fmax are IEEE on both sides. */
 #include "oracle.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -617,6 +618,95 @@ int or_trace_pixels(const or_scene* S, const or_frame* F, const int* px, const i
     work();
     for (auto& t : th) t.join();
     return nthreads;
+}
+
+extern "C++" {
+namespace {
+
+// Runs f(i) for i in [0, n) on nthreads threads (dynamic scheduling in chunks of 16).
+template <class Fn>
+void parallel_for(int n, int nthreads, Fn f) {
+    if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
+    if (nthreads <= 0) nthreads = 1;
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            int b = next.fetch_add(16);
+            if (b >= n) break;
+            int e = b + 16 < n ? b + 16 : n;
+            for (int i = b; i < e; ++i) f(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthreads; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+}  // namespace
+}  // extern "C++"
+
+long long or_render_adaptive(const or_scene* S, const or_frame* F, int spp, int spp_max, float target_err,
+                             int jitter, int nthreads, double* out_px, uint32_t* out_nsamp) {
+    const int W = F->W, H = F->H, n = W * H, chunk = 8;
+    const int lum = S->mode == 1 ? 0 : 1;
+    std::vector<double> acc((size_t)n * 4, 0.0);
+    std::vector<float> ys((size_t)n, 0.f), yq((size_t)n, 0.f);
+    auto sample = [&](int p, int s, double* a, float* sy, float* sq) {
+        double tx, ty;
+        or_sample_offsets(F, p % W, p / W, s, jitter, &tx, &ty);
+        or_ray_out o;
+        or_trace_ray(S, F, tx, ty, &o);
+        for (int c = 0; c < 4; ++c) a[c] += o.I[c];
+        if (sy) {
+            const float Y = (float)o.I[lum];
+            *sy += Y;
+            *sq += Y * Y;
+        }
+    };
+    parallel_for(n, nthreads, [&](int p) {
+        for (int s = 0; s < spp; ++s) sample(p, s, &acc[4 * (size_t)p], &ys[p], &yq[p]);
+    });
+    std::vector<uint32_t> chunks((size_t)n, 0u);
+    const int extra_max = spp_max - spp > 0 ? spp_max - spp : 0;
+    long long total = 0;
+    if (extra_max > 0) {
+        double fsum = 0.0;
+        for (int b = 0; b < n; b += 256) {
+            double part = 0.0;
+            for (int p = b; p < n && p < b + 256; ++p) part += (double)ys[p];
+            fsum += part;
+        }
+        const float yref = (float)(0.03 * fsum / ((double)n * spp) + 1e-37);
+        std::vector<uint32_t> extra((size_t)n);
+        for (int p = 0; p < n; ++p) {   // render.cu:130-139
+            const float m = (float)spp;
+            const float mean = ys[p] / m, var = std::fmax(yq[p] / m - mean * mean, 0.f);
+            const float se = std::sqrt(var / m), den = target_err * (mean + yref);
+            const float want = m * (se / den) * (se / den);
+            extra[p] = (uint32_t)std::fmin(std::fmax(std::ceil(want) - m, 0.f), (float)extra_max);
+        }
+        for (int y = 0; y < H; ++y)   // render.cu:141-150
+            for (int x = 0; x < W; ++x) {
+                uint32_t mx = 0;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int yy = std::min(std::max(y + dy, 0), H - 1), xx = std::min(std::max(x + dx, 0), W - 1);
+                        mx = std::max(mx, extra[(size_t)yy * W + xx]);
+                    }
+                chunks[(size_t)y * W + x] = (std::min(mx, (uint32_t)extra_max) + chunk - 1) / chunk;
+            }
+        for (int p = 0; p < n; ++p) total += chunks[p];
+        parallel_for(n, nthreads, [&](int p) {
+            for (uint32_t k = 0; k < chunks[p] * chunk; ++k) sample(p, spp + (int)k, &acc[4 * (size_t)p], nullptr, nullptr);
+        });
+    }
+    for (int p = 0; p < n; ++p) {
+        const uint32_t m = (uint32_t)spp + chunks[p] * chunk;
+        for (int c = 0; c < 4; ++c) out_px[4 * (size_t)p + c] = acc[4 * (size_t)p + c] / m;
+        if (out_nsamp) out_nsamp[p] = m;
+    }
+    return (long long)n * spp + total * chunk;
 }
 
 void or_math_eval(int fn, const double* x, const double* y, double* out, int n) {

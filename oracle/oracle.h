@@ -112,6 +112,20 @@ void or_trace_ray(const or_scene* S, const or_frame* F, double x, double yv, or_
 int or_trace_pixels(const or_scene* S, const or_frame* F, const int* px, const int* py, int n, int spp,
                     int jitter, int nthreads, double* out_px, or_ray_out* out_rays);
 
+// Adaptive sampling of a W x H frame (F->W, F->H), restating render.cu:89-242 and 677-721 with this
+// project's deterministic sums:
+//  - pass 1: samples 0..spp-1 of every pixel; per pixel the FP64 sum of I[0..3] in sample order and
+//    the float luminance sums ys += Y, yq += Y*Y with Y = (float)I[0] (radio) or (float)I[1] (visible);
+//  - yref = (float)(0.03 * fsum / (n * spp) + 1e-37), fsum = FP64 sum of the ys in blocks of 256
+//    consecutive pixels (each block in pixel order, then the blocks in order);
+//  - extra samples per pixel as k_adapt (float arithmetic, render.cu:130-139), dilated by the 3 x 3
+//    maximum and rounded up to chunks of 8 (render.cu:141-150, at most spp_max - spp);
+//  - pass 2: samples spp .. spp + 8 * chunks - 1 of each pixel, added to the pass-1 sum in order;
+//  - pixel value = FP64 sum / sample count.
+// out_px: W*H*4 doubles; out_nsamp (nullable): W*H sample counts. Returns the total sample count.
+long long or_render_adaptive(const or_scene* S, const or_frame* F, int spp, int spp_max, float target_err,
+                             int jitter, int nthreads, double* out_px, uint32_t* out_nsamp);
+
 // Elementary function fn (numbering of bh_math_eval in include/bhrt.h) on n inputs, with this
 // build's functions (glibc or bh_math.h); 9 = sqrt, 10 = x / y.
 void or_math_eval(int fn, const double* x, const double* y, double* out, int n);

@@ -101,6 +101,9 @@ class Oracle:
         L.or_table_lerp.argtypes = [_F, C.c_int, C.c_float, C.c_float, C.c_float]
         L.or_sample_offsets.argtypes = [C.POINTER(OrFrame), C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]
         L.or_k2_table.argtypes = [C.c_int, _F, _F, _F]
+        L.or_render_adaptive.restype = C.c_longlong
+        L.or_render_adaptive.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.c_int, C.c_int, C.c_float, C.c_int,
+                                         C.c_int, _D, C.POINTER(C.c_uint32)]
         L.or_turb_factor.restype = C.c_double
         L.or_turb_factor.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_flow_step.argtypes = [C.POINTER(OrScene)] + [C.c_double] * 7 + [_D, _D]
@@ -213,6 +216,16 @@ class Oracle:
 
     def turb_factor(self, S, r, u, phi, t=0.0):
         return self.lib.or_turb_factor(C.byref(S), r, u, phi, t)
+
+    def render_adaptive(self, S, F, spp, spp_max, target_err, jitter=True, nthreads=0):
+        """(image H x W x 4 float64, samples per pixel H x W, total samples)."""
+        W, H = F.W, F.H
+        img = np.zeros((H, W, 4), dtype=np.float64)
+        ns = np.zeros((H, W), dtype=np.uint32)
+        total = self.lib.or_render_adaptive(C.byref(S), C.byref(F), spp, spp_max, C.c_float(target_err), int(jitter),
+                                            nthreads, img.ctypes.data_as(_D),
+                                            ns.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return img, ns, total
 
     def flow_step(self, S, r, u, phi, t, lam, Ecam, h):
         dI = C.c_double()

@@ -23,7 +23,7 @@ EXPORTS = [
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
-    "bh_trace_batch", "bh_autotune",
+    "bh_trace_batch", "bh_autotune", "bh_render_adaptive",
 ]
 MLP_NPARAM = 8706
 MLP_NNORM = 10   # input centres[3], half-widths[3], output means[2], scales[2]
@@ -163,6 +163,9 @@ def load():
     L.bh_mlp_grad.argtypes = [C.c_void_p, _F, _F, _F, _F, _F, C.c_int, _F, _F]
     L.bh_mlp_bake.restype = C.c_int
     L.bh_mlp_bake.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(MlpBakeReport)]
+    L.bh_render_adaptive.restype = C.c_int
+    L.bh_render_adaptive.argtypes = [C.c_void_p, C.POINTER(Frame), C.POINTER(TraceOpts), _F, C.POINTER(C.c_uint32),
+                                     C.POINTER(Stats)]
     L.bh_trace_batch.restype = C.c_int
     L.bh_trace_batch.argtypes = [C.c_void_p, C.POINTER(BatchFrame), C.c_int, C.POINTER(TraceOpts),
                                  C.POINTER(BatchCfg), _F, C.POINTER(Stats)]
@@ -401,6 +404,17 @@ class Tracer:
         self._check(self.L.bh_render(self.ctx, C.byref(F), C.byref(o), px.ctypes.data_as(_F), C.byref(st)),
                     "bh_render")
         return px, st
+
+    def render_adaptive(self, F, opts=None):
+        """(image H x W x 4, samples per pixel H x W, Stats) with the context's spp / spp_max / target_err."""
+        px = np.zeros((F.H, F.W, 4), dtype=np.float32)
+        ns = np.zeros((F.H, F.W), dtype=np.uint32)
+        st = Stats()
+        o = opts or self.opts()
+        self._check(self.L.bh_render_adaptive(self.ctx, C.byref(F), C.byref(o), px.ctypes.data_as(_F),
+                                              ns.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(st)),
+                    "bh_render_adaptive")
+        return px, ns, st
 
     def render_resident(self, F, opts=None):
         st = Stats()

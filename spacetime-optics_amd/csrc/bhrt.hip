@@ -104,8 +104,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, const bhm3
                 q->accepted = r.accepted;
                 q->flow_steps = r.flow_steps;
             } else {
-                const double inv = 1.0 / (double)P.spp;
-                px_out[o] = make_float4((float)(a0 * inv), (float)(a1 * inv), (float)(a2 * inv), (float)(a3 * inv));
+                const double m = (double)P.spp;   // mean as sum / count, as the adaptive resolve and the oracle
+                px_out[o] = make_float4((float)(a0 / m), (float)(a1 / m), (float)(a2 / m), (float)(a3 / m));
             }
         }
     }
@@ -157,13 +157,12 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
 
 // WAVES > 1 asks the compiler for at least that many resident waves per SIMD (fewer VGPRs, possibly
 // spills); 1 leaves the register budget to the compiler.
-template <int PREC, bool RADIO, int WAVES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_persist(const bhp::BatchFrame* __restrict__ frames, int nframes,
-                                                   unsigned int* __restrict__ queue, int interval, int refill_min,
-                                                   BlockOut* __restrict__ blk) {
+template <int PREC, bool RADIO, int WAVES, class Work>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_persist(
+    const Work wk, unsigned int* __restrict__ queue, int interval, int refill_min, BlockOut* __restrict__ blk) {
     double flux;
     unsigned long long c[NCNT];
-    bhp::persist_body<PREC, RADIO>(frames, nframes, queue, interval, refill_min, &flux, c);
+    bhp::persist_body<PREC, RADIO>(wk, queue, interval, refill_min, &flux, c);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __shared__ double s_flux[BLOCK / 64];
     __shared__ unsigned long long s_cnt[BLOCK / 64][NCNT];
@@ -184,6 +183,153 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) 
         }
         blk[blockIdx.x] = b;
     }
+}
+
+/* ---- adaptive sampling (render.cu:89-242, 677-721), deterministic form ----
+   Pass-1 samples are kept per (pixel, sample) and summed per pixel in sample order; the frame sum of
+   the luminance goes through fixed blocks of 256 pixels; pass-2 samples are kept per entry and added
+   in order. The decisions (extra samples per pixel) use the reference's float formulas. */
+constexpr int ADAPT_CHUNK = 8;     // samples per pass-2 work item (render.cu:30)
+constexpr int FSUM_BLOCK = 256;    // pixels per partial of the frame luminance sum
+
+// Per pixel: FP64 sums of the pass-1 samples (sample order) and the float luminance sums.
+__global__ void k_adapt_stats(const double4* __restrict__ samp, int npix, int spp, int lum,
+                              double4* __restrict__ acc, float2* __restrict__ ystat) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    float ys = 0.f, yq = 0.f;
+    for (int s = 0; s < spp; ++s) {
+        const double4 v = samp[(size_t)p * spp + s];
+        a0 += v.x;
+        a1 += v.y;
+        a2 += v.z;
+        a3 += v.w;
+        const float Y = (float)(lum == 0 ? v.x : v.y);
+        ys += Y;
+        yq += Y * Y;
+    }
+    acc[p] = make_double4(a0, a1, a2, a3);
+    ystat[p] = make_float2(ys, yq);
+}
+
+// Partial luminance sums over FSUM_BLOCK consecutive pixels, each in pixel order.
+__global__ void k_adapt_fsum(const float2* __restrict__ ystat, int npix, double* __restrict__ part) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p0 = b * FSUM_BLOCK;
+    if (p0 >= npix) return;
+    double s = 0.0;
+    for (int p = p0; p < npix && p < p0 + FSUM_BLOCK; ++p) s += (double)ystat[p].x;
+    part[b] = s;
+}
+
+// render.cu:130-139 with yref = 0.03 * (frame luminance sum) / (npix * spp) + 1e-37 (render.cu:702).
+// The partials are summed in order by every thread (a few thousand L2-resident loads).
+__global__ void k_adapt_extra(const float2* __restrict__ ystat, int npix, int spp, const double* __restrict__ part,
+                              int nparts, float target, int extra_max, unsigned int* __restrict__ extra) {
+    __shared__ float s_yref;
+    if (threadIdx.x == 0) {
+        double fsum = 0.0;
+        for (int b = 0; b < nparts; ++b) fsum += part[b];
+        s_yref = (float)(0.03 * fsum / ((double)npix * spp) + 1e-37);
+    }
+    __syncthreads();
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float n = (float)spp;
+    const float2 st = ystat[p];
+    const float mean = st.x / n, var = fmaxf(st.y / n - mean * mean, 0.f);
+    const float se = sqrtf(var / n), den = target * (mean + s_yref);
+    const float want = n * (se / den) * (se / den);
+    extra[p] = (unsigned int)fminf(fmaxf(ceilf(want) - n, 0.f), (float)extra_max);
+}
+
+// render.cu:141-150: 3 x 3 maximum, clamped at the edges, in chunks of ADAPT_CHUNK samples.
+__global__ void k_adapt_dilate(const unsigned int* __restrict__ extra, int W, int H, int extra_max,
+                               unsigned int* __restrict__ chunks) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    unsigned int m = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+            m = max(m, extra[(size_t)min(max(y + dy, 0), H - 1) * W + min(max(x + dx, 0), W - 1)]);
+    chunks[(size_t)y * W + x] = (min(m, (unsigned int)extra_max) + ADAPT_CHUNK - 1) / ADAPT_CHUNK;
+}
+
+// Exclusive prefix sum of unsigned ints: per-1024 block scans (wave64 shuffles, then the 16 wave
+// totals), the block totals scanned by one workgroup in rounds of 1024, then added back.
+__device__ __forceinline__ unsigned int wave_incl_scan(unsigned int v, int lane) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned int block_excl_scan(unsigned int v, unsigned int* total) {
+    __shared__ unsigned int ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned int inc = wave_incl_scan(v, lane);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    if (w == 0) {   // one wave scans the 16 wave totals in place (its reads precede its writes)
+        unsigned int t = lane < 16 ? ws[lane] : 0u;
+        t = wave_incl_scan(t, lane);
+        if (lane < 16) ws[lane] = t;
+    }
+    __syncthreads();
+    const unsigned int pre = w > 0 ? ws[w - 1] : 0u;
+    *total = ws[15];
+    __syncthreads();   // ws is rewritten by the next call
+    return pre + inc - v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_block(const unsigned int* __restrict__ in, unsigned int* __restrict__ out,
+                                                     unsigned int* __restrict__ bsum, int n) {
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    unsigned int tot;
+    const unsigned int ex = block_excl_scan(i < n ? in[i] : 0u, &tot);
+    if (i < n) out[i] = ex;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(unsigned int* __restrict__ bsum, int nb) {
+    unsigned int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        unsigned int tot;
+        const unsigned int ex = block_excl_scan(i < nb ? bsum[i] : 0u, &tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_add(unsigned int* __restrict__ out, const unsigned int* __restrict__ bsum, int n) {
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+// Pixel = (pass-1 sum + pass-2 samples in order) / sample count.
+__global__ void k_adapt_resolve(const double4* __restrict__ acc, const double4* __restrict__ samp2,
+                                const unsigned int* __restrict__ offsets, const unsigned int* __restrict__ chunks,
+                                int npix, int spp, float4* __restrict__ out, unsigned int* __restrict__ nsamp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    double4 a = acc[p];
+    const unsigned int k = chunks ? chunks[p] : 0u;
+    if (k) {
+        const double4* v = samp2 + (size_t)offsets[p] * ADAPT_CHUNK;
+        for (unsigned int i = 0; i < k * ADAPT_CHUNK; ++i) {
+            a.x += v[i].x;
+            a.y += v[i].y;
+            a.z += v[i].z;
+            a.w += v[i].w;
+        }
+    }
+    const unsigned int m = (unsigned int)spp + k * ADAPT_CHUNK;
+    const double dm = (double)m;
+    out[p] = make_float4((float)(a.x / dm), (float)(a.y / dm), (float)(a.z / dm), (float)(a.w / dm));
+    if (nsamp) nsamp[p] = m;
 }
 
 // graph-safe reset of the work queues (a kernel, not a memset node)
@@ -495,12 +641,21 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     return BH_OK;
 }
 
-using PersistFn = void (*)(const bhp::BatchFrame*, int, unsigned int*, int, int, BlockOut*);
+using PersistFn = void (*)(const bhp::BatchWork, unsigned int*, int, int, BlockOut*);
 
 template <int W>
 PersistFn pick_persist_w(int prec, bool radio) {
-    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, W> : k_persist<BH_PREC_FP64, false, W>;
-    return radio ? k_persist<BH_PREC_MIXED, true, W> : k_persist<BH_PREC_MIXED, false, W>;
+    using BW = bhp::BatchWork;
+    if (prec == BH_PREC_FP64)
+        return radio ? k_persist<BH_PREC_FP64, true, W, BW> : k_persist<BH_PREC_FP64, false, W, BW>;
+    return radio ? k_persist<BH_PREC_MIXED, true, W, BW> : k_persist<BH_PREC_MIXED, false, W, BW>;
+}
+
+// adaptive passes: kernels compiled for 3 waves per SIMD (the autotuned choice on both precisions)
+template <class Work>
+void (*pick_adaptive(int prec, bool radio))(const Work, unsigned int*, int, int, BlockOut*) {
+    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, 3, Work> : k_persist<BH_PREC_FP64, false, 3, Work>;
+    return radio ? k_persist<BH_PREC_MIXED, true, 3, Work> : k_persist<BH_PREC_MIXED, false, 3, Work>;
 }
 
 PersistFn pick_persist(int prec, bool radio, int waves) {
@@ -591,8 +746,8 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
-        hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), 0, c->st, dframes, n, c->d_queue, cfg.interval,
-                           cfg.refill_min, c->d_blk);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), 0, c->st, bhp::BatchWork{dframes, n}, c->d_queue,
+                           cfg.interval, cfg.refill_min, c->d_blk);
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
         return hipGetLastError();
     };
@@ -646,6 +801,122 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
         st->step_limited = (long long)tot.cnt[3];
         st->fp64_steps = (long long)tot.cnt[4];
         st->flux_sum = tot.flux;
+    }
+    return BH_OK;
+}
+
+// Adaptive render (render.cu:677-721): persistent pass 1 over (tile, sample, pixel) entries, the
+// statistics and decisions on the device, one host read of the pass-2 size, persistent pass 2 over
+// chunks, resolve. px_out / nsamp_out are device pointers (d_px, d_nsamp).
+int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, long long* samples, bh_stats* st) {
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    rc = check_opts(c, o);
+    if (rc) return rc;
+    const int spp = o.spp > 0 ? o.spp : (c->rs.spp > 0 ? c->rs.spp : 8);
+    const int spp_max = c->rs.spp_max > 0 ? c->rs.spp_max : 128;
+    const float target = c->rs.target_err > 0.f ? c->rs.target_err : 0.03f;
+    const int W = f->W, H = f->H, npix = W * H;
+    const int extra_max = std::max(spp_max - spp, 0);
+    const int tiles_x = (W + 7) / 8, tiles = tiles_x * ((H + 7) / 8);
+    if ((long long)tiles * 64 * spp >= (1ll << 31) ||
+        (long long)npix * ((extra_max + ADAPT_CHUNK - 1) / ADAPT_CHUNK) * ADAPT_CHUNK >= (1ll << 31))
+        return fail(c, BH_ERR_ARG, "frame too large for adaptive sampling (2^31 samples)");
+    bh_batch_cfg cfg = c->tuned_ok ? c->tuned : batch_defaults();
+    const int grid = c->n_cu * cfg.grid_per_cu;
+    bhp::BatchFrame B[2];
+    std::memset(B, 0, sizeof B);
+    B[0].P = make_params(c, f);
+    B[0].P.w = W;
+    B[0].P.h = H;
+    B[0].P.spp = spp;
+    B[0].P.jitter = o.jitter;
+    B[0].M = make_mixed(c, c->S, o);
+    if (o.emissivity == BH_EMIS_MLP) {
+        B[0].M.use_surrogate = 1;
+        B[0].M.S = c->stab;
+    }
+    B[0].tiles_x = tiles_x;
+    B[0].nentries = (unsigned int)(tiles * 64 * spp);
+    B[1] = B[0];
+    const size_t n = (size_t)npix;
+    if ((rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof B)) || (rc = ensure(c, &c->d_queue, &c->queue_cap, 1)) ||
+        (rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid)) ||
+        (rc = ensure(c, &c->d_samp1, &c->samp1_cap, n * spp)) || (rc = ensure(c, &c->d_acc, &c->acc_cap, n)) ||
+        (rc = ensure(c, &c->d_ystat, &c->ystat_cap, n)) || (rc = ensure(c, &c->d_extra, &c->extra_cap, n)) ||
+        (rc = ensure(c, &c->d_chunks, &c->chunks_cap, n)) || (rc = ensure(c, &c->d_offsets, &c->offsets_cap, n)) ||
+        (rc = ensure(c, &c->d_bsum, &c->bsum_cap, (n + 1023) / 1024)) ||
+        (rc = ensure(c, &c->d_fpart, &c->fpart_cap, (n + FSUM_BLOCK - 1) / FSUM_BLOCK)) ||
+        (rc = ensure(c, &c->d_nsamp, &c->nsamp_cap, n)) || (rc = ensure(c, &c->d_px, &c->px_cap, n)))
+        return rc;
+    const bhp::BatchFrame* dB = (const bhp::BatchFrame*)c->d_bframes;
+    const bool radio = c->S.mode == BH_MODE_RADIO;
+    HIPCK(c, hipEventRecord(c->ev0, c->st));
+    HIPCK(c, hipMemcpyAsync(c->d_bframes, B, sizeof B, hipMemcpyHostToDevice, c->st));
+    // pass 1
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
+    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
+                       bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, c->d_blk);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
+    HIPCK(c, hipGetLastError());
+    BlockOut tot1, tot2{};
+    HIPCK(c, hipMemcpyAsync(&tot1, c->d_tot, sizeof tot1, hipMemcpyDeviceToHost, c->st));
+    const int lum = radio ? 0 : 1;
+    hipLaunchKernelGGL(k_adapt_stats, dim3((npix + 255) / 256), dim3(256), 0, c->st, c->d_samp1, npix, spp, lum,
+                       c->d_acc, c->d_ystat);
+    unsigned int total = 0;
+    if (extra_max > 0) {
+        const int nparts = (npix + FSUM_BLOCK - 1) / FSUM_BLOCK;
+        hipLaunchKernelGGL(k_adapt_fsum, dim3((nparts + 255) / 256), dim3(256), 0, c->st, c->d_ystat, npix, c->d_fpart);
+        hipLaunchKernelGGL(k_adapt_extra, dim3((npix + 255) / 256), dim3(256), 0, c->st, c->d_ystat, npix, spp,
+                           c->d_fpart, nparts, target, extra_max, c->d_extra);
+        hipLaunchKernelGGL(k_adapt_dilate, dim3((W + 15) / 16, (H + 15) / 16), dim3(16, 16), 0, c->st, c->d_extra, W, H,
+                           extra_max, c->d_chunks);
+        const int nb = (npix + 1023) / 1024;
+        hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(1024), 0, c->st, c->d_chunks, c->d_offsets, c->d_bsum, npix);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->st, c->d_bsum, nb);
+        hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, c->st, c->d_offsets, c->d_bsum, npix);
+        HIPCK(c, hipGetLastError());
+        unsigned int last[2];
+        HIPCK(c, hipMemcpyAsync(&last[0], c->d_offsets + npix - 1, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipMemcpyAsync(&last[1], c->d_chunks + npix - 1, 4, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipStreamSynchronize(c->st));
+        total = last[0] + last[1];
+        if (total > 0) {
+            rc = ensure(c, &c->d_samp2, &c->samp2_cap, (size_t)total * ADAPT_CHUNK);
+            if (rc) return rc;
+            B[1].nentries = total * ADAPT_CHUNK;
+            HIPCK(c, hipMemcpyAsync(c->d_bframes + sizeof(bhp::BatchFrame), &B[1], sizeof B[1], hipMemcpyHostToDevice,
+                                    c->st));
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
+            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
+                               bhp::ChunkWork{dB + 1, 1, spp, npix, c->d_offsets, c->d_samp2}, c->d_queue,
+                               cfg.interval, cfg.refill_min, c->d_blk);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
+            HIPCK(c, hipGetLastError());
+            HIPCK(c, hipMemcpyAsync(&tot2, c->d_tot, sizeof tot2, hipMemcpyDeviceToHost, c->st));
+        }
+    }
+    hipLaunchKernelGGL(k_adapt_resolve, dim3((npix + 255) / 256), dim3(256), 0, c->st, c->d_acc, c->d_samp2,
+                       c->d_offsets, total > 0 ? c->d_chunks : nullptr, npix, spp, c->d_px, c->d_nsamp);
+    HIPCK(c, hipGetLastError());
+    HIPCK(c, hipEventRecord(c->ev1, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    const long long ns = (long long)npix * spp + (long long)total * ADAPT_CHUNK;
+    if (samples) *samples = ns;
+    if (st) {
+        float ms = 0.f;
+        HIPCK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->ms_kernel = ms;
+        st->samples = ns;
+        st->step_attempts = (long long)(tot1.cnt[0] + tot2.cnt[0]);
+        st->steps_accepted = (long long)(tot1.cnt[1] + tot2.cnt[1]);
+        st->flow_steps = (long long)(tot1.cnt[2] + tot2.cnt[2]);
+        st->step_limited = (long long)(tot1.cnt[3] + tot2.cnt[3]);
+        st->fp64_steps = (long long)(tot1.cnt[4] + tot2.cnt[4]);
+        st->flux_sum = tot1.flux + tot2.flux;
     }
     return BH_OK;
 }
@@ -768,6 +1039,23 @@ int bh_render(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, float* px
         HIPCK(c, hipMemcpyAsync(px_out, c->d_px, sizeof(float4) * (size_t)f->W * f->H, hipMemcpyDeviceToHost, c->st));
         HIPCK(c, hipStreamSynchronize(c->st));
     }
+    local.ms_trace = now_ms() - t0;
+    if (st) *st = local;
+    return BH_OK;
+}
+
+int bh_render_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, float* px_out,
+                       unsigned int* nsamp_out, bh_stats* st) {
+    if (!c) return BH_ERR_ARG;
+    double t0 = now_ms();
+    HIPCK(c, hipSetDevice(c->device));
+    bh_stats local{};
+    int rc = launch_adaptive(c, f, opts, nullptr, &local);
+    if (rc) return rc;
+    const size_t n = (size_t)f->W * f->H;
+    if (px_out) HIPCK(c, hipMemcpyAsync(px_out, c->d_px, sizeof(float4) * n, hipMemcpyDeviceToHost, c->st));
+    if (nsamp_out) HIPCK(c, hipMemcpyAsync(nsamp_out, c->d_nsamp, sizeof(unsigned int) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
     local.ms_trace = now_ms() - t0;
     if (st) *st = local;
     return BH_OK;
@@ -950,8 +1238,10 @@ void bh_destroy(bh_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph_def) (void)hipGraphDestroy(c->graph_def);
-    void* bufs[] = {c->d_k2,  c->d_turb,  c->d_px,  c->d_rays,   c->d_blk,    c->d_tot,  c->d_ph,    c->d_fates,
-                    c->d_mlp, c->d_adam_m, c->d_adam_v, c->d_stab, c->d_sbad, c->d_bframes, c->d_queue, c->d_bout};
+    void* bufs[] = {c->d_k2,     c->d_turb,   c->d_px,     c->d_rays,    c->d_blk,    c->d_tot,    c->d_ph,
+                    c->d_fates,  c->d_mlp,    c->d_adam_m, c->d_adam_v,  c->d_stab,   c->d_sbad,   c->d_bframes,
+                    c->d_queue,  c->d_bout,   c->d_samp1,  c->d_acc,     c->d_ystat,  c->d_extra,  c->d_chunks,
+                    c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -62,6 +62,19 @@ struct bh_ctx {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     unsigned long long graph_key = 0;
+    // adaptive sampling: per-sample pass-1 values, per-pixel sums and statistics, decisions, pass 2
+    double4* d_samp1 = nullptr;
+    size_t samp1_cap = 0;
+    double4* d_acc = nullptr;
+    size_t acc_cap = 0;
+    float2* d_ystat = nullptr;
+    size_t ystat_cap = 0;
+    unsigned int *d_extra = nullptr, *d_chunks = nullptr, *d_offsets = nullptr, *d_bsum = nullptr, *d_nsamp = nullptr;
+    size_t extra_cap = 0, chunks_cap = 0, offsets_cap = 0, bsum_cap = 0, nsamp_cap = 0;
+    double* d_fpart = nullptr;
+    size_t fpart_cap = 0;
+    double4* d_samp2 = nullptr;
+    size_t samp2_cap = 0;
     int n_cu = 0;
     bh_batch_cfg tuned{};   // last bh_autotune result (used when bh_trace_batch gets no cfg)
     bool tuned_ok = false;

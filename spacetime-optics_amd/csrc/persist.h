@@ -8,7 +8,9 @@
    its rank among the free lanes. A wave refills only from its current frame and moves to the next
    frame once its lanes are drained, so the frame parameters stay wave-uniform (scalar loads).
    Every wave exits when all frames are exhausted and its lanes are idle; each ray runs at most
-   max_steps steps, so the grid always drains. */
+   max_steps steps, so the grid always drains. What an entry means (which pixel and sample, where
+   the finished ray is written) is a Work policy: frame batches, and the two passes of adaptive
+   sampling. */
 #pragma once
 #include "trace_fp64.h"
 #include "trace_mixed.h"
@@ -21,22 +23,86 @@ using bhd::TraceParams;
 struct BatchFrame {
     TraceParams P;         // scene + frame; P.w/P.h = frame size, P.spp = 1
     bhm32::MixedParams M;  // the frame's mixed-precision policy and emission constants
-    unsigned int nentries; // 64 * (8x8 tiles)
+    unsigned int nentries; // queue entries of the frame
     int tiles_x;
-    float4* out;           // W*H pixels
+    float4* out;           // W*H pixels (frame batches)
 };
 
-struct BatchCounters {
-    double flux;
-    unsigned long long cnt[5];
-};
-
-__device__ __forceinline__ bool entry_pixel(const BatchFrame& B, unsigned int e, int* px, int* py) {
-    const unsigned int tile = e >> 6, within = e & 63u;
-    *px = (int)(tile % (unsigned)B.tiles_x) * 8 + (int)(within & 7u);
-    *py = (int)(tile / (unsigned)B.tiles_x) * 8 + (int)(within >> 3);
-    return *px < B.P.w && *py < B.P.h;
+__device__ __forceinline__ void tile_pixel(int tiles_x, unsigned int tile, unsigned int within, int* px, int* py) {
+    *px = (int)(tile % (unsigned)tiles_x) * 8 + (int)(within & 7u);
+    *py = (int)(tile / (unsigned)tiles_x) * 8 + (int)(within >> 3);
 }
+
+/* Work mappings: entry e of frame f -> (pixel, sample index) and where a finished ray goes.
+   Entries of 64 consecutive indices are 8 x 8 pixel tiles, so a wave's rays are image neighbours. */
+
+// Frame batches: one pixel-centre (or jittered sample 0) ray per pixel, written as a float4 pixel.
+struct BatchWork {
+    const BatchFrame* frames;
+    int n;
+    __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
+        tile_pixel(frames[f].tiles_x, e >> 6, e & 63u, px, py);
+        *s = 0;
+        return *px < frames[f].P.w && *py < frames[f].P.h;
+    }
+    __device__ __forceinline__ void finish(int f, unsigned int e, const RayResult& r) const {
+        int px, py;
+        tile_pixel(frames[f].tiles_x, e >> 6, e & 63u, &px, &py);
+        frames[f].out[(size_t)py * frames[f].P.w + px] =
+            make_float4((float)r.I[0], (float)r.I[1], (float)r.I[2], (float)r.I[3]);
+    }
+};
+
+// First pass of adaptive sampling: samples 0..spp-1 of every pixel (entry = tile, sample, pixel in
+// tile), each kept in FP64 at out[pixel * spp + sample].
+struct SampleWork {
+    const BatchFrame* frames;   // one frame
+    int n;
+    int spp;
+    double4* out;
+    __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
+        const unsigned int per_tile = 64u * (unsigned)spp, rem = e % per_tile;
+        tile_pixel(frames[f].tiles_x, e / per_tile, rem & 63u, px, py);
+        *s = (int)(rem >> 6);
+        return *px < frames[f].P.w && *py < frames[f].P.h;
+    }
+    __device__ __forceinline__ void finish(int f, unsigned int e, const RayResult& r) const {
+        const unsigned int per_tile = 64u * (unsigned)spp, rem = e % per_tile;
+        int px, py;
+        tile_pixel(frames[f].tiles_x, e / per_tile, rem & 63u, &px, &py);
+        out[((size_t)py * frames[f].P.w + px) * spp + (rem >> 6)] = make_double4(r.I[0], r.I[1], r.I[2], r.I[3]);
+    }
+};
+
+// Second pass: chunk g of the exclusive prefix sum `offsets` of per-pixel chunk counts belongs to
+// the last pixel p with offsets[p] <= g; its 8 samples continue p's sequence at
+// spp + 8 * (g - offsets[p]). Each sample is kept in FP64 at out[entry].
+struct ChunkWork {
+    const BatchFrame* frames;   // one frame
+    int n;
+    int spp;
+    int npix;
+    const unsigned int* offsets;
+    double4* out;
+    __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
+        const unsigned int g = e >> 3;
+        int lo = 0, hi = npix;   // upper bound of g in offsets
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (offsets[mid] <= g) lo = mid + 1;
+            else hi = mid;
+        }
+        const int p = lo - 1;
+        const int W = frames[f].P.w;
+        *px = p % W;
+        *py = p / W;
+        *s = spp + 8 * (int)(g - offsets[p]) + (int)(e & 7u);
+        return true;
+    }
+    __device__ __forceinline__ void finish(int, unsigned int e, const RayResult& r) const {
+        out[e] = make_double4(r.I[0], r.I[1], r.I[2], r.I[3]);
+    }
+};
 
 template <int PREC, bool RADIO>
 struct Tracer;
@@ -72,16 +138,16 @@ struct Tracer<BH_PREC_MIXED, RADIO> {
     }
 };
 
-template <int PREC, bool RADIO>
-__device__ __forceinline__ void persist_body(const BatchFrame* __restrict__ frames, int nframes,
-                                             unsigned int* __restrict__ queue, int interval, int refill_min,
-                                             double* flux_out,
-                                             unsigned long long* cnt_out) {
+template <int PREC, bool RADIO, class Work>
+__device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __restrict__ queue, int interval,
+                                             int refill_min, double* flux_out, unsigned long long* cnt_out) {
     using T = Tracer<PREC, RADIO>;
+    const BatchFrame* __restrict__ frames = Wk.frames;
+    const int nframes = Wk.n;
     const int lane = threadIdx.x & 63;
     int cur = 0;   // the wave's frame (wave-uniform)
     bool active = false;
-    int px = 0, py = 0;
+    unsigned int ent = 0;   // the lane's queue entry (the work mapping recovers pixel and sample)
     typename T::State R;
     double flux = 0.0;
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
@@ -103,11 +169,13 @@ __device__ __forceinline__ void persist_body(const BatchFrame* __restrict__ fram
                     if (!active) {
                         const unsigned int rank = (unsigned int)__popcll(freem & ((1ull << lane) - 1ull));
                         const unsigned int e = base + rank;
-                        if (e < n && entry_pixel(frames[cur], e, &px, &py)) {
+                        int px, py, s = 0;
+                        if (e < n && Wk.begin(cur, e, &px, &py, &s)) {
                             const TraceParams& P = frames[cur].P;
                             double ox, oy;
-                            bhd::sample_offsets(P, px, py, 0, &ox, &oy);
+                            bhd::sample_offsets(P, px, py, s, &ox, &oy);
                             T::init(P, frames[cur].M, ox, oy, R);
+                            ent = e;
                             active = true;
                         }
                     }
@@ -130,8 +198,7 @@ __device__ __forceinline__ void persist_body(const BatchFrame* __restrict__ fram
                 if (R.done) {
                     RayResult r;
                     const int f64 = T::result(R, r);
-                    frames[cf].out[(size_t)py * P.w + px] =
-                        make_float4((float)r.I[0], (float)r.I[1], (float)r.I[2], (float)r.I[3]);
+                    Wk.finish(cf, ent, r);
                     flux += r.I[0];
                     c0 += (unsigned long long)r.steps;
                     c1 += (unsigned long long)r.accepted;
