@@ -87,9 +87,10 @@ typedef struct bh_trace_opts {
     int spp;          /* samples per pixel; 0 -> frame->spp */
     double tol_far;   /* mixed: tolerance of the FP32 far-field steps (0 -> 1e-6) */
     double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
-    double r_emit;    /* mixed, radio: FP64 at the scene tolerance below max(r_emit, r_near), FP32 stages
-                         outside (0 -> the radius outside which the scene's flow model, in its mid-plane,
-                         emits less than 1e-6 of its r^2-weighted emissivity) */
+    double r_emit;    /* mixed, radio: the scene tolerance below max(r_emit, r_near), tol_far above; FP64
+                         stages inside the flow, FP32 outside (0 -> the radius outside which the scene's
+                         flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
+                         emissivity) */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

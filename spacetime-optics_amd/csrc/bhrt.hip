@@ -533,6 +533,7 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_trace_opts&
     const double r_ph = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::min(1.0, std::fabs(S.a)))));
     const double r_near = o.r_near > 0 ? o.r_near : std::max(3.0 * S.rh, r_ph) + 1.0;
     M.r_far = r_near;
+    M.r_tol = r_near;
     if (S.mode == BH_MODE_RADIO && S.flow_on) {
         double r_emit = o.r_emit;
         if (!(r_emit > 0)) {
@@ -549,7 +550,8 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_trace_opts&
                 c->emit_cache.emplace_back(k, r_emit);
             }
         }
-        M.r_far = std::max(std::min(r_emit, S.r_flow_max), r_near);
+        M.r_far = std::max(S.r_flow_max, r_near);
+        M.r_tol = std::max(std::min(r_emit, S.r_flow_max), r_near);
     }
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);

@@ -1,15 +1,21 @@
 /* trace_mixed.h - mixed FP32/FP64 per-ray tracer for gfx950 (device code, included by bhrt.hip).
 
    Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame):
-   - Far field (r >= r_far: outside the emitting flow in radio mode, outside r_near in vacuum mode):
+   - Far field (r >= r_far: outside the flow in radio mode, outside r_near in vacuum mode):
      DP5(4) stages in FP32 (FP32 reciprocals instead of divisions), the state y accumulated in FP64,
      the error norm, step controller and radial-constraint projection in FP64, tolerance tol_far.
      The projection r' = +-sqrt(R(r)) after every accepted step (FP64) keeps the radial motion on
      the photon's constant-of-motion surface, so FP32 stage errors only shift where along the
      trajectory a step lands.
-   - Inside the flow and near the hole: FP64 geodesic at the scene tolerance (inside the flow the
-     reference's steps are tolerance-limited, and the radiative-transfer quadrature depends on that
-     step sequence), evaluated with fused multiply-adds and Newton-refined FP64 reciprocals.
+   - Inside the flow (radio) or inside r_near (vacuum): FP64 geodesic stages, evaluated with fused
+     multiply-adds and Newton-refined FP64 reciprocals. The tolerance is the scene's inside r_tol
+     (radio: the radius holding all but 1e-6 of the flow's emission; near the hole in vacuum) -
+     there the reference's steps are tolerance-limited and the radiative-transfer quadrature
+     depends on that step sequence - and tol_far between r_tol and the flow's edge, where the
+     emission is negligible and the steps are set by the transfer step cap. The precision switch
+     sits at the flow's edge, the tolerance switch inside it: neighbouring rays of a wave cross
+     r_tol at different times, and a per-lane tolerance costs nothing, while a per-lane precision
+     would run both stage codes for the whole wave.
    - Emission and absorption in FP32, in log space (the cgs coefficients span 1e-45..1e-10), exact
      thermal-synchrotron formula or the MLP surrogate table; intensity and transmission accumulate
      in FP64. The swept angle (photon-ring order) is tracked in FP32.
@@ -303,6 +309,7 @@ __device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T,
 
 struct MixedParams {
     double r_far;     // far field (FP32 stages) at r >= r_far
+    double r_tol;     // scene tolerance below r_tol, tol_far above (FP64 stages up to r_far)
     double tol_far;   // tolerance of the far-field steps
     EmisParams E;
     int use_surrogate;
@@ -371,7 +378,8 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
             R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
         }
-        err = dp54<double>(cd, R.s, hc, P.tol, P.tol4, n);
+        const double tol = R.s.y0 < M.r_tol ? P.tol : M.tol_far;
+        err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, n);
         ++R.fp64_attempts;
     }
     R.k1_f32 = R.s.y0 >= M.r_far;
