@@ -1,12 +1,16 @@
 /* bhrt - command line front end on the MI355X tracer (blackholes/src/main.cpp:374-626 of the
    reference, single GPU): bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B]
-   [--list] [--selftest]. Frames are written as linear HDR (I, I_n1, I_n2, I_n>=3) float images
-   "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first; tone mapping, HUD and PNG output are not
-   part of this path. */
+   [--list] [--selftest]. Frames are rendered as the reference's Renderer::render does (adaptive
+   sampling between render.spp and render.spp_max) and written as linear HDR (I, I_n1, I_n2,
+   I_n>=3) float images "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first; tone mapping, HUD
+   and PNG output are not part of this path. A shot with a [batch] section (frames, spins, sizes,
+   frame_dt_m) is traced in one persistent launch instead: frame i has spin spins[i % #spins],
+   size sizes[(i / #spins) % #sizes] squared and time i * frame_dt_m. */
 #include <sys/stat.h>
 
 #include <chrono>
 #include <cmath>
+#include <sstream>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -160,6 +164,73 @@ int main(int argc, char** argv) {
     o.emissivity = opt.emissivity;
     o.jitter = opt.jitter;
     o.spp = rs.spp;
+    if (opt.emissivity == BH_EMIS_MLP) {
+        bh_mlp_report mr{};
+        bh_mlp_bake_report br{};
+        if (bh_mlp_train(R.ctx(), nullptr, &mr) != BH_OK || bh_mlp_bake(R.ctx(), 0, 0, 0, 0.f, &br) != BH_OK) {
+            std::fprintf(stderr, "bhrt: surrogate: %s\n", R.error().c_str());
+            return 1;
+        }
+        std::printf("  emissivity surrogate: trained in %.1f s (loss %.3g -> %.3g), %lld of %lld hot cells judged exact-only\n",
+                    mr.ms_train * 1e-3, mr.loss_first, mr.loss_last, br.failed_hot_cells, br.hot_cells);
+    }
+    if (cfg.has("batch.frames")) {
+        // one persistent launch over the [batch] frames
+        auto list = [&](const std::string& key, const std::string& dflt) {
+            std::vector<double> v;
+            std::stringstream ss(cfg.str(key, dflt));
+            std::string item;
+            while (std::getline(ss, item, ','))
+                if (!item.empty()) v.push_back(std::atof(item.c_str()));
+            return v;
+        };
+        const int nb = cfg.integer("batch.frames", 1);
+        const std::vector<double> spins = list("batch.spins", std::to_string(S.a));
+        const std::vector<double> sizes = list("batch.sizes", std::to_string(rs.W));
+        const double dt = cfg.num("batch.frame_dt_m", 0.0);
+        if (spins.empty() || sizes.empty() || nb <= 0) {
+            std::fprintf(stderr, "bhrt: bad [batch] section\n");
+            return 1;
+        }
+        std::vector<bh_batch_frame> B((size_t)nb);
+        for (int i = 0; i < nb; ++i) {
+            bh::Config ci = cfg;
+            ci.set("bh.spin", std::to_string(spins[(size_t)i % spins.size()]));
+            bh_scene Si;
+            bh::SceneTables Ti;
+            bh::SceneInfo sii;
+            bh::TracerOptions oi;
+            if (!bh::build_scene(ci, &Si, &Ti, &sii, &oi, &err, false)) {
+                std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+                return 1;
+            }
+            Si.n0 = S.n0;   // the normalised density of the base scene
+            const int n = (int)sizes[((size_t)i / spins.size()) % sizes.size()];
+            bh::ShotFrame sf = shot.frames[0];
+            sf.time_m = i * dt;
+            B[(size_t)i].scene = Si;
+            B[(size_t)i].frame = bh::make_frame(shot, sf, Si.a, n, n, 1, 0x9E3779B9u * (uint32_t)(i + 1));
+        }
+        std::vector<bh::HdrFrame> imgs;
+        bh_stats st{};
+        double t1 = wall();
+        if (!R.render_batch(B, o, &imgs, &st)) {
+            std::fprintf(stderr, "bhrt: batch: %s\n", R.error().c_str());
+            return 1;
+        }
+        const double tb = wall() - t1;
+        for (int i = 0; i < nb; ++i) {
+            char name[64];
+            std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", i);
+            if (!write_frame(out + name, imgs[(size_t)i])) {
+                std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+                return 1;
+            }
+        }
+        std::printf("done: batch of %d frames, %.3g rays in %.1f ms kernel / %.2f s wall (%.1f Mrays/s)\n", nb,
+                    (double)st.samples, st.ms_kernel, tb, st.samples / std::max(st.ms_kernel, 1e-9) * 1e-3);
+        return 0;
+    }
     long long samples = 0, limited = 0;
     double t_render = wall();
     for (int idx = fa; idx < fb; ++idx) {

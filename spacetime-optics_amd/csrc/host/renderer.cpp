@@ -36,7 +36,28 @@ bool Renderer::render(const bh_frame& f, const bh_trace_opts& opts, HdrFrame* ou
     out->W = f.W;
     out->H = f.H;
     out->px.resize((size_t)f.W * f.H * 4);
+    if (rs_.spp_max > rs_.spp) return bh_render_adaptive(ctx_, &f, &opts, out->px.data(), nullptr, st) == BH_OK;
     return bh_render(ctx_, &f, &opts, out->px.data(), st) == BH_OK;
+}
+
+bool Renderer::render_batch(const std::vector<bh_batch_frame>& frames, const bh_trace_opts& opts,
+                            std::vector<HdrFrame>* out, bh_stats* st) {
+    if (!ctx_ || frames.empty()) return false;
+    size_t total = 0;
+    for (const bh_batch_frame& b : frames) total += (size_t)b.frame.W * b.frame.H;
+    std::vector<float> px(total * 4);
+    if (bh_trace_batch(ctx_, frames.data(), (int)frames.size(), &opts, nullptr, px.data(), st) != BH_OK) return false;
+    out->resize(frames.size());
+    size_t off = 0;
+    for (size_t i = 0; i < frames.size(); ++i) {
+        HdrFrame& h = (*out)[i];
+        h.W = frames[i].frame.W;
+        h.H = frames[i].frame.H;
+        const size_t n = (size_t)h.W * h.H * 4;
+        h.px.assign(px.begin() + off, px.begin() + off + n);
+        off += n;
+    }
+    return true;
 }
 
 double Renderer::radio_flux(bh_frame f, int n, double pix_sr) {

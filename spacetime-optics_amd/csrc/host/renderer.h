@@ -26,8 +26,14 @@ public:
 
     bool init(int device, const bh_scene& scene, const SceneTables& tables, const bh_settings& rs, std::string* err);
     bool update_scene(const bh_scene& s);
-    // Frame trace (no adaptive sampling or post); opts selects precision / emissivity / jitter.
+    // Renderer::render (render.cu:755-770) without post: adaptive sampling when the settings allow
+    // more than spp samples (spp_max > spp), else the plain mean of spp samples. opts selects
+    // precision / emissivity / jitter.
     bool render(const bh_frame& f, const bh_trace_opts& opts, HdrFrame* out, bh_stats* st);
+    // One persistent launch over several frames (each with its own scene): pixel-centre or sample-0
+    // rays, images one after the other in `out`.
+    bool render_batch(const std::vector<bh_batch_frame>& frames, const bh_trace_opts& opts,
+                      std::vector<HdrFrame>* out, bh_stats* st);
     // Flux density (Jy) of an n x n, spp = 2 jittered quick render; < 0 on error.
     double radio_flux(bh_frame f, int n, double pix_sr);
     bool vacuum_fates(const std::vector<double>& photons, double tol, double r_escape, int max_steps,
