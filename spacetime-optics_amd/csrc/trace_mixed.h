@@ -3,7 +3,8 @@
    Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame):
    - Far field (r >= r_far: outside the flow in radio mode, outside r_near in vacuum mode):
      DP5(4) stages in FP32 (FP32 reciprocals instead of divisions), the state y accumulated in FP64,
-     the error norm, step controller and radial-constraint projection in FP64, tolerance tol_far.
+     the error norm, step controller and radial-constraint projection in FP64, tolerance tol_far
+     (radio: scaled with the pixel's size, 1e-4 at 0.04 M pixels; vacuum: 1e-6).
      The projection r' = +-sqrt(R(r)) after every accepted step (FP64) keeps the radial motion on
      the photon's constant-of-motion surface, so FP32 stage errors only shift where along the
      trajectory a step lands.
@@ -11,7 +12,7 @@
      multiply-adds and Newton-refined FP64 reciprocals. The tolerance is the scene's inside r_tol
      (radio: the radius holding all but 1e-6 of the flow's emission; near the hole in vacuum) -
      there the reference's steps are tolerance-limited and the radiative-transfer quadrature
-     depends on that step sequence - and tol_far between r_tol and the flow's edge, where the
+     depends on that step sequence - and tol_mid between r_tol and the flow's edge, where the
      emission is negligible and the steps are set by the transfer step cap. The precision switch
      sits at the flow's edge, the tolerance switch inside it: neighbouring rays of a wave cross
      r_tol at different times, and a per-lane tolerance costs nothing, while a per-lane precision
@@ -309,7 +310,8 @@ __device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T,
 
 struct MixedParams {
     double r_far;     // far field (FP32 stages) at r >= r_far
-    double r_tol;     // scene tolerance below r_tol, tol_far above (FP64 stages up to r_far)
+    double r_tol;     // scene tolerance below r_tol, tol_mid above (FP64 stages up to r_far)
+    double tol_mid;   // tolerance of the FP64 steps between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
     EmisParams E;
     int use_surrogate;
@@ -378,7 +380,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
             R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
         }
-        const double tol = R.s.y0 < M.r_tol ? P.tol : M.tol_far;
+        const double tol = R.s.y0 < M.r_tol ? P.tol : M.tol_mid;
         err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, n);
         ++R.fp64_attempts;
     }
