@@ -86,7 +86,8 @@ typedef struct bh_trace_opts {
     int jitter;       /* 0: pixel-centre samples; 1: R2 + PCG jitter as render.cu:70-85 */
     int spp;          /* samples per pixel; 0 -> frame->spp */
     double tol_far;   /* mixed: tolerance of the FP32 far-field steps (0 -> vacuum: max(1e-6, tol);
-                         radio: 1e-4 * pixel / 0.04 M, clamped to [max(1e-6, tol), 1e-4]) */
+                         radio: 1e-4 * (pix * r) / 0.04 M, the pixel's size at the hole, clamped to
+                         [max(1e-6, tol), 1e-4]) */
     double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
     double r_emit;    /* mixed, radio: the scene tolerance below max(r_emit, r_near), tol_far above; FP64
                          stages inside the flow, FP32 outside (0 -> the radius outside which the scene's

@@ -534,11 +534,12 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
     M.tol_far = M.tol_mid;
     if (o.tol_far > 0) {
         M.tol_far = o.tol_far;
-    } else if (radio && f && f->pix > 0) {
+    } else if (radio && f && f->pix > 0 && f->r > 0) {
         // far-field error that stays far below a pixel: 1e-4 on 0.04 M pixels (1024^2 over 40 M, where
         // CPU sweeps against the FP64 oracle give 6e-7 flux and 2e-5 median pixel error), scaled
-        // with the pixel size and clamped to [max(1e-6, tol), 1e-4]
-        M.tol_far = std::min(1e-4, std::max(M.tol_mid, 1e-4 * f->pix / 0.04));
+        // with the pixel size at the hole (pix is an angle: pix * r in M) and clamped to
+        // [max(1e-6, tol), 1e-4]
+        M.tol_far = std::min(1e-4, std::max(M.tol_mid, 1e-4 * (f->pix * f->r) / 0.04));
     }
     const double r_ph = 2.0 * (1.0 + std::cos(2.0 / 3.0 * std::acos(std::min(1.0, std::fabs(S.a)))));
     const double r_near = o.r_near > 0 ? o.r_near : std::max(3.0 * S.rh, r_ph) + 1.0;

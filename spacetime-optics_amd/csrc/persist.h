@@ -190,8 +190,11 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
         }
         if (__ballot(active) == 0ull && cur >= nframes) break;
         const int cf = __builtin_amdgcn_readfirstlane(cur < nframes ? cur : nframes - 1);
-        const TraceParams& P = frames[cf].P;
-        const bhm32::MixedParams& M = frames[cf].M;
+        // frame parameters through the constant address space: wave-uniform scalar loads
+        using CFrame = const __attribute__((address_space(4))) BatchFrame;
+        CFrame* cfr = (CFrame*)frames;
+        const TraceParams& P = *(const TraceParams*)&cfr[cf].P;
+        const bhm32::MixedParams& M = *(const bhm32::MixedParams*)&cfr[cf].M;
         for (int k = 0; k < interval; ++k) {
             if (active) {
                 if (!R.done) T::step(P, M, R);

@@ -24,7 +24,7 @@ def test_adaptive_matches_oracle(oracle_bhm):
     cfg = setup(shot("c2_radio_1024.ini"), W=24, H=24)
     cfg["F"].seed = cfg["OF"].seed = 0x1234567
     t = tracer(cfg, 4, 36, 0.03)
-    img, ns, st = t.render_adaptive(cfg["F"], t.opts(jitter=1))
+    img, ns, st = t.render_adaptive(cfg["F"], t.opts(jitter=1, spp=4))
     ref, ns_ref, total = oracle_bhm.render_adaptive(cfg["OS"], cfg["OF"], 4, 36, 0.03, jitter=True)
     print(f"samples per pixel: {np.bincount(ns.ravel())[4::8]} (4, 12, 20, ...); total {st.samples}")
     assert np.array_equal(ns, ns_ref)
@@ -39,7 +39,7 @@ def test_adaptive_equals_plain_mean_over_same_samples(prec):
     cfg = setup(shot("c2_radio_1024.ini"), W=128, H=128)
     cfg["F"].seed = 0xBEEF
     t = tracer(cfg, 8, 64, 0.03)
-    img, ns, st = t.render_adaptive(cfg["F"], t.opts(precision=prec, jitter=1))
+    img, ns, st = t.render_adaptive(cfg["F"], t.opts(precision=prec, jitter=1, spp=8))
     assert st.samples == int(ns.sum())
     counts = np.unique(ns)
     print(f"prec {prec}: sample counts {dict(zip(counts.tolist(), np.bincount(ns.ravel())[counts].tolist()))}, "
@@ -56,7 +56,7 @@ def test_adaptive_vacuum_takes_no_extra_samples():
     """Visible mode drives the decision with channel 1, which the vacuum tracer leaves at zero."""
     cfg = setup(shot("c1_vacuum_256.ini"), W=64, H=64)
     t = tracer(cfg, 2, 32, 0.03)
-    img, ns, st = t.render_adaptive(cfg["F"], t.opts(jitter=1))
+    img, ns, st = t.render_adaptive(cfg["F"], t.opts(jitter=1, spp=2))
     assert (ns == 2).all() and st.samples == 2 * 64 * 64
     plain, _ = t.render(cfg["F"], t.opts(jitter=1, spp=2))
     assert np.array_equal(img, plain)

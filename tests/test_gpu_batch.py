@@ -127,7 +127,9 @@ def test_graph_replay_with_eager_work_between(c2):
         again, st = t.trace_batch(frames, o)
         assert_same(again, first, "replay")
         assert np.array_equal(stats_vec(st), stats_vec(st0))
-        assert st.flux_sum == st0.flux_sum
+        # which rays a workgroup retires is decided at run time, so the per-workgroup flux partials
+        # (and the order of the frame sum) vary between launches: equal to rounding
+        assert abs(st.flux_sum - st0.flux_sum) <= 1e-12 * abs(st0.flux_sum)
 
 
 def test_autotune(c2):
