@@ -249,12 +249,18 @@ def main():
     head = results[head_mode]
     fp64 = results.get("fp64")
 
-    traffic = None
+    # bytes the dominant kernel moves between L2 and memory per launch, from the committed counter
+    # profile of the same mode (FETCH_SIZE + WRITE_SIZE, calibrated on a 1 GiB copy)
+    traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(head_mode)
-        except (OSError, ValueError):
+            t = json.load(open(tpath)).get(head_mode)
+            if t:
+                traffic = int(t["read_bytes"] + t["write_bytes"])
+                traffic_note = (f"{t['kernel']}: {t['read_bytes'] / 1e6:.1f} MB read + {t['write_bytes'] / 1e6:.1f} MB "
+                                f"written per launch ({t['counters']}; profiles/c2_radio_1024_counters.md)")
+        except (OSError, ValueError, KeyError):
             traffic = None
 
     cpu = None
@@ -281,6 +287,7 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(head["achieved_tflops"], 3),
                      "peak": round(head["peak_tflops"], 2), "unit": "TFLOP/s", "frac": round(head["roof_frac"], 4),
                      "traffic": traffic,
+                     "traffic_note": traffic_note,
                      "note": "vector-ALU roof (FP64 78.6, FP32 157.3 TFLOP/s, weighted by the work each precision "
                              "did); work = 600 flop-units per DP54 step attempt + 200 per in-flow step, counted "
                              "by the kernel; HIP-event kernel time"},
