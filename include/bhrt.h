@@ -89,10 +89,12 @@ typedef struct bh_trace_opts {
                          radio: 1e-4 * (pix * r) / 0.04 M, the pixel's size at the hole, clamped to
                          [max(1e-6, tol), 1e-4]) */
     double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
-    double r_emit;    /* mixed, radio: the scene tolerance below max(r_emit, r_near), tol_far above; FP64
-                         stages inside the flow, FP32 outside (0 -> the radius outside which the scene's
-                         flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
+    double r_emit;    /* mixed, radio: the scene tolerance below max(r_emit, r_near), max(1e-6, tol) from
+                         there to the flow's edge, tol_far beyond (0 -> the radius outside which the
+                         scene's flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
                          emissivity) */
+    int far_fp64;     /* mixed: 0 -> FP32 stages in the far field (radio: outside the flow; vacuum:
+                         outside r_near); 1 -> FP64 stages everywhere, same tolerances */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

@@ -309,9 +309,10 @@ __device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T,
 }
 
 struct MixedParams {
-    double r_far;     // far field (FP32 stages) at r >= r_far
-    double r_tol;     // scene tolerance below r_tol, tol_mid above (FP64 stages up to r_far)
-    double tol_mid;   // tolerance of the FP64 steps between r_tol and r_far
+    double r_far;     // far field (tolerance tol_far) at r >= r_far
+    double r_f32;     // FP32 stages at r >= r_f32 (>= r_far; infinite: FP64 stages everywhere)
+    double r_tol;     // scene tolerance below r_tol, tol_mid between r_tol and r_far
+    double tol_mid;   // tolerance between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
     EmisParams E;
     int use_surrogate;
@@ -333,7 +334,7 @@ template <bool RADIO>
 __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedParams& M, double x, double yv,
                                            RayStateM& R) {
     bhd::camera_photon(P, x, yv, R.c, R.s);
-    R.k1_f32 = R.s.y0 >= M.r_far;
+    R.k1_f32 = R.s.y0 >= M.r_f32;
     if (R.k1_f32) {
         const RConst<float> cf = make_const<float>(P, R.c);
         Rhs<float> k = rhs<float>(cf, (float)R.s.y0, (float)R.s.y2);
@@ -371,7 +372,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         hc = fmin(hc, cap);
     }
     double err;
-    if (R.s.y0 >= M.r_far) {
+    if (R.s.y0 >= M.r_f32) {
         const RConst<float> cf = make_const<float>(P, R.c);
         err = dp54<float>(cf, R.s, hc, M.tol_far, M.tol_far * 4.0, n);
     } else {
@@ -380,11 +381,11 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
             R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
         }
-        const double tol = R.s.y0 < M.r_tol ? P.tol : M.tol_mid;
+        const double tol = R.s.y0 < M.r_tol ? P.tol : (R.s.y0 < M.r_far ? M.tol_mid : M.tol_far);
         err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, n);
         ++R.fp64_attempts;
     }
-    R.k1_f32 = R.s.y0 >= M.r_far;
+    R.k1_f32 = R.s.y0 >= M.r_f32;
     ++R.it;
     if (!(err <= 1.0)) {
         R.h = hc * fmax(0.2, 0.9 * pow_m02_fast(err));
