@@ -93,8 +93,10 @@ typedef struct bh_trace_opts {
                          there to the flow's edge, tol_far beyond (0 -> the radius outside which the
                          scene's flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
                          emissivity) */
-    int far_fp64;     /* mixed: 0 -> FP32 stages in the far field (radio: outside the flow; vacuum:
-                         outside r_near); 1 -> FP64 stages everywhere, same tolerances */
+    int far_fp32;     /* mixed: 0 -> FP64 stages everywhere (default: on MI355X FP32 arithmetic without
+                         packed FP32 runs at the FP64 rate, and a per-lane precision switch costs
+                         both stage codes per wave); 1 -> FP32 stages in the far field (radio: outside
+                         the flow; vacuum: outside r_near), same tolerances */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

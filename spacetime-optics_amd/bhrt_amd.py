@@ -63,7 +63,7 @@ class Settings(C.Structure):
 
 class TraceOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
-                ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp64", C.c_int)]
+                ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -395,8 +395,8 @@ class Tracer:
 
     @staticmethod
     def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0, r_emit=0.0,
-             far_fp64=0):
-        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp64)
+             far_fp32=0):
+        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)

@@ -564,7 +564,7 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
         M.r_far = std::max(S.r_flow_max, r_near);
         M.r_tol = std::max(std::min(r_emit, S.r_flow_max), r_near);
     }
-    M.r_f32 = o.far_fp64 ? 1e300 : M.r_far;
+    M.r_f32 = o.far_fp32 ? M.r_far : 1e300;
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);
     E.n_index = (float)S.n_index;

@@ -1,22 +1,21 @@
 /* trace_mixed.h - mixed FP32/FP64 per-ray tracer for gfx950 (device code, included by bhrt.hip).
 
-   Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame):
-   - Far field (r >= r_far: outside the flow in radio mode, outside r_near in vacuum mode):
-     DP5(4) stages in FP32 (FP32 reciprocals instead of divisions), the state y accumulated in FP64,
-     the error norm, step controller and radial-constraint projection in FP64, tolerance tol_far
-     (radio: scaled with the pixel's size, 1e-4 at 0.04 M pixels; vacuum: 1e-6).
-     The projection r' = +-sqrt(R(r)) after every accepted step (FP64) keeps the radial motion on
+   Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame and from
+   timings on MI355X):
+   - Tolerance regions: the scene tolerance inside r_tol (radio: the radius holding all but 1e-6 of
+     the flow's emission; vacuum: r_near, near the hole) - there the reference's steps are
+     tolerance-limited and the radiative-transfer quadrature depends on that step sequence;
+     tol_mid from r_tol to r_far (the flow's edge in radio mode), where the emission is negligible
+     and the steps are set by the transfer step cap; tol_far beyond (radio: scaled with the
+     pixel's size, 1e-4 on 0.04 M pixels; vacuum: 1e-6).
+   - Stage precision: FP64 stages evaluated with fused multiply-adds and Newton-refined FP64
+     reciprocals; optionally (r >= r_f32) DP5(4) stages in FP32 with hardware reciprocals, the
+     state accumulated in FP64, the error norm, step controller and radial-constraint projection
+     in FP64. The projection r' = +-sqrt(R(r)) after every accepted step keeps the radial motion on
      the photon's constant-of-motion surface, so FP32 stage errors only shift where along the
-     trajectory a step lands.
-   - Inside the flow (radio) or inside r_near (vacuum): FP64 geodesic stages, evaluated with fused
-     multiply-adds and Newton-refined FP64 reciprocals. The tolerance is the scene's inside r_tol
-     (radio: the radius holding all but 1e-6 of the flow's emission; near the hole in vacuum) -
-     there the reference's steps are tolerance-limited and the radiative-transfer quadrature
-     depends on that step sequence - and tol_mid between r_tol and the flow's edge, where the
-     emission is negligible and the steps are set by the transfer step cap. The precision switch
-     sits at the flow's edge, the tolerance switch inside it: neighbouring rays of a wave cross
-     r_tol at different times, and a per-lane tolerance costs nothing, while a per-lane precision
-     would run both stage codes for the whole wave.
+     trajectory a step lands. The FP32 stages are off by default: without packed FP32 the
+     arithmetic runs at the FP64 rate, and lanes of a wave crossing r_f32 at different steps run
+     both stage codes.
    - Emission and absorption in FP32, in log space (the cgs coefficients span 1e-45..1e-10), exact
      thermal-synchrotron formula or the MLP surrogate table; intensity and transmission accumulate
      in FP64. The swept angle (photon-ring order) is tracked in FP32.

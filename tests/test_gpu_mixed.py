@@ -19,14 +19,16 @@ def frame_errors(a, b):
     return flux, float(np.median(pix)), float(np.percentile(pix, 99))
 
 
-def test_mixed_radio_c2_error_bounds():
+@pytest.mark.parametrize("far_fp32", [0, 1])
+def test_mixed_radio_c2_error_bounds(far_fp32):
+    """Both stage-precision policies: FP64 stages everywhere (default) and FP32 stages outside the flow."""
     cfg = setup(shot("c2_radio_1024.ini"))
     cfg["S"].n0 = 291464.1043528089   # the 0.6 Jy normalisation of this frame
     t = tracer_for(cfg)
     ref, st64 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_FP64))
-    mix, st32 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_MIXED))
+    mix, st32 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_MIXED, far_fp32=far_fp32))
     flux, med, p99 = frame_errors(mix, ref)
-    print(f"mixed vs fp64: flux {flux:.3e} median {med:.3e} p99 {p99:.3e}; steps/ray "
+    print(f"mixed (far_fp32={far_fp32}) vs fp64: flux {flux:.3e} median {med:.3e} p99 {p99:.3e}; steps/ray "
           f"{st32.step_attempts / 2**20:.1f} vs {st64.step_attempts / 2**20:.1f}, FP64 attempts "
           f"{st32.fp64_steps / st32.step_attempts:.2%}; kernel {st32.ms_kernel:.2f} vs {st64.ms_kernel:.2f} ms")
     assert flux < 1e-3
@@ -38,16 +40,17 @@ def test_mixed_radio_c2_error_bounds():
         assert abs(a - b) < 2e-2 * b
 
 
-def test_mixed_vacuum_c1_fates():
+@pytest.mark.parametrize("far_fp32", [0, 1])
+def test_mixed_vacuum_c1_fates(far_fp32):
     cfg = setup(shot("c1_vacuum_256.ini"))
     t = tracer_for(cfg)
     a = t.trace_tile(cfg["F"], 0, 0, 256, 256, bh.Tracer.opts(precision=bh.PREC_FP64)).ravel()
-    b = t.trace_tile(cfg["F"], 0, 0, 256, 256, bh.Tracer.opts(precision=bh.PREC_MIXED)).ravel()
+    b = t.trace_tile(cfg["F"], 0, 0, 256, 256, bh.Tracer.opts(precision=bh.PREC_MIXED, far_fp32=far_fp32)).ravel()
     same_fate = (a["fate"] == b["fate"]).mean()
     same_n = (a["ncross"] == b["ncross"]).mean()
     esc = (a["fate"] == 1) & (b["fate"] == 1)
     dth = np.abs(a["th_inf"][esc] - b["th_inf"][esc])
-    print(f"vacuum mixed: fate agreement {same_fate:.5f}, ncross {same_n:.5f}, |dtheta_inf| median "
+    print(f"vacuum mixed (far_fp32={far_fp32}): fate agreement {same_fate:.5f}, ncross {same_n:.5f}, |dtheta_inf| median "
           f"{np.median(dth):.2e} max {dth.max():.2e}, steps {b['steps'].mean():.1f} vs {a['steps'].mean():.1f}")
     assert same_fate > 0.999
     assert same_n > 0.995
