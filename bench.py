@@ -175,6 +175,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="time the persistent modes with the default launch")
     ap.add_argument("--no-mlp-train", action="store_true", help="skip MLP modes")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a device copy before the modes (profiling)")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-sampling render")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (rays shard with no "
@@ -265,6 +266,19 @@ def main():
         except (OSError, ValueError, KeyError):
             traffic = None
 
+    # bhrt's production render of this frame (adaptive passes, spp 8 -> up to 128, jittered), for reference
+    adaptive = {}
+    if not args.no_adaptive:
+        for name, prec in (("fp64", bh.PREC_FP64), ("mixed", bh.PREC_MIXED)):
+            o = bh.Tracer.opts(precision=prec, jitter=1, spp=8)
+            tracer.render_adaptive(F, o)   # warm-up (first launch of these kernels)
+            _, ns, st = tracer.render_adaptive(F, o)
+            adaptive[name] = {"ms": round(st.ms_kernel, 2), "samples": int(st.samples),
+                              "msamples_per_s": round(st.samples / st.ms_kernel * 1e-3, 2),
+                              "pixels_refined": int((ns > 8).sum())}
+            log(f"bench: adaptive render ({name}): {st.ms_kernel:.1f} ms, {st.samples / 1e6:.2f} M samples "
+                f"({st.samples / st.ms_kernel * 1e-3:.1f} Msamples/s), {int((ns > 8).sum())} pixels refined")
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_pixels} pixels")
@@ -296,6 +310,7 @@ def main():
         "cpu_baseline": cpu,
         "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for m, r in results.items()},
         "setup": setup,
+        "adaptive_render_spp8_128": adaptive or None,
     }
     if fp64:
         line["speedup_vs_fp64"] = round(head["mrays_per_s"] / fp64["mrays_per_s"], 3)
