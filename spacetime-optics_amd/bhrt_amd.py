@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libbhrt.so")
 BH_OK, BH_ERR_ARG, BH_ERR_HIP, BH_ERR_NODEVICE, BH_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 MODE_VISIBLE, MODE_RADIO = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+ORDER_ROWS, ORDER_IMPACT = 1, 2
 EMIS_EXACT, EMIS_MLP = 0, 1
 
 # Every symbol declared in include/bhrt.h and include/bhrt_host.h
@@ -83,7 +84,7 @@ class BatchFrame(C.Structure):
 
 class BatchCfg(C.Structure):
     _fields_ = [("block", C.c_int), ("grid_per_cu", C.c_int), ("interval", C.c_int), ("refill_min", C.c_int),
-                ("use_graph", C.c_int), ("waves", C.c_int)]
+                ("use_graph", C.c_int), ("waves", C.c_int), ("order", C.c_int), ("prio_age", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -454,7 +455,7 @@ class Tracer:
         return arr
 
     def trace_batch(self, frames, opts=None, cfg=None, block=0, grid_per_cu=0, interval=0, refill_min=0,
-                    use_graph=1, waves=0, resident=False):
+                    use_graph=1, waves=0, order=0, prio_age=0, resident=False):
         """frames: list of (Scene, Frame). Returns ([H x W x 4 images], Stats), or (None, Stats) with
         resident=True (the images stay in device memory). cfg None and all launch arguments 0 -> the
         last autotuned configuration, else the defaults."""
@@ -463,8 +464,9 @@ class Tracer:
         px = np.zeros((0 if resident else npix, 4), dtype=np.float32)
         st = Stats()
         o = opts or self.opts()
-        if cfg is None and (block or grid_per_cu or interval or refill_min or waves or use_graph != 1):
-            cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph, waves)
+        if cfg is None and (block or grid_per_cu or interval or refill_min or waves or order or prio_age or
+                            use_graph != 1):
+            cfg = BatchCfg(block, grid_per_cu, interval, refill_min, use_graph, waves, order, prio_age)
         pc = C.byref(cfg) if cfg is not None else None
         dst = None if resident else px.ctypes.data_as(_F)
         self._check(self.L.bh_trace_batch(self.ctx, arr, len(frames), C.byref(o), pc, dst, C.byref(st)),
@@ -481,7 +483,7 @@ class Tracer:
         arr = self._batch(frames)
         best = BatchCfg()
         ms = C.c_double()
-        tried = np.zeros((max_tried, 6), dtype=np.float64)
+        tried = np.zeros((max_tried, 8), dtype=np.float64)
         nt = C.c_int()
         o = opts or self.opts()
         self._check(self.L.bh_autotune(self.ctx, arr, len(frames), C.byref(o), C.byref(best), C.byref(ms),

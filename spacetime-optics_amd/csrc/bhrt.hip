@@ -159,10 +159,11 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
 // spills); 1 leaves the register budget to the compiler.
 template <int PREC, bool RADIO, int WAVES, class Work>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_persist(
-    const Work wk, unsigned int* __restrict__ queue, int interval, int refill_min, BlockOut* __restrict__ blk) {
+    const Work wk, unsigned int* __restrict__ queue, int interval, int refill_min, int prio_age,
+    BlockOut* __restrict__ blk) {
     double flux;
     unsigned long long c[NCNT];
-    bhp::persist_body<PREC, RADIO>(wk, queue, interval, refill_min, &flux, c);
+    bhp::persist_body<PREC, RADIO>(wk, queue, interval, refill_min, prio_age, &flux, c);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __shared__ double s_flux[BLOCK / 64];
     __shared__ unsigned long long s_cnt[BLOCK / 64][NCNT];
@@ -336,6 +337,68 @@ __global__ void k_adapt_resolve(const double4* __restrict__ acc, const double4* 
 __global__ void k_zero_u32(unsigned int* __restrict__ p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+/* Queue order of a persistent launch: the tiles of each frame sorted by the impact parameter b of
+   their centre ray, smallest first. b^2 = eta + lam^2 (+ a^2 cos^2 theta_cam, the same for the whole
+   frame) is the distance from the hole's image for a distant camera; rays near and inside the
+   critical curve cross the flow and orbit longest, and starting them first keeps them from being
+   the last rays of the launch. Counting sort over ORDER_BUCKETS bins of b (ORDER_BINS_PER_M per M,
+   the last bin open-ended); the order inside a bin is whatever the atomics give, which changes no
+   pixel. SCATTER = false counts the bins, true writes each tile at its bin's next position. */
+constexpr int ORDER_BUCKETS = 1024;
+constexpr double ORDER_BINS_PER_M = 16.0;
+
+template <bool SCATTER>
+__global__ __launch_bounds__(256) void k_order_tiles(const bhp::BatchFrame* __restrict__ frames,
+                                                     unsigned int* __restrict__ hist) {
+    const int f = blockIdx.y;
+    const bhp::BatchFrame& F = frames[f];
+    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (F.nentries >> 6)) return;
+    int px, py;
+    bhp::tile_pixel(F.tiles_x, t, 36u, &px, &py);   // pixel (4, 4) of the 8 x 8 tile
+    px = min(px, F.P.w - 1);
+    py = min(py, F.P.h - 1);
+    double ox, oy;
+    bhd::sample_offsets(F.P, px, py, 0, &ox, &oy);
+    bhd::RayConst rc;
+    bhd::State st;
+    bhd::camera_photon(F.P, ox, oy, rc, st);
+    const double b = sqrt(fmax(rc.eta + rc.lam * rc.lam, 0.0));
+    const unsigned int k = (unsigned int)fmin(b * ORDER_BINS_PER_M, (double)(ORDER_BUCKETS - 1));
+    unsigned int* h = hist + (size_t)f * ORDER_BUCKETS + k;
+    if (!SCATTER) {
+        atomicAdd(h, 1u);
+    } else {
+        F.order[atomicAdd(h, 1u)] = t;
+    }
+}
+
+// Exclusive prefix sum of each frame's ORDER_BUCKETS bin counts, in place (one workgroup per frame).
+__global__ __launch_bounds__(256) void k_order_scan(unsigned int* __restrict__ hist) {
+    static_assert(ORDER_BUCKETS == 4 * 256, "4 bins per thread");
+    unsigned int* h = hist + (size_t)blockIdx.x * ORDER_BUCKETS;
+    __shared__ unsigned int part[256];
+    const int t = threadIdx.x;
+    unsigned int v[4], sum = 0u;
+    for (int i = 0; i < 4; ++i) {
+        v[i] = h[4 * t + i];
+        sum += v[i];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned int x = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    unsigned int run = part[t] - sum;
+    for (int i = 0; i < 4; ++i) {
+        h[4 * t + i] = run;
+        run += v[i];
+    }
 }
 
 __global__ __launch_bounds__(64) void k_vacuum_fates(const TraceParams P, const double* __restrict__ photons, int n,
@@ -655,7 +718,7 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     return BH_OK;
 }
 
-using PersistFn = void (*)(const bhp::BatchWork, unsigned int*, int, int, BlockOut*);
+using PersistFn = void (*)(const bhp::BatchWork, unsigned int*, int, int, int, BlockOut*);
 
 template <int W>
 PersistFn pick_persist_w(int prec, bool radio) {
@@ -667,7 +730,7 @@ PersistFn pick_persist_w(int prec, bool radio) {
 
 // adaptive passes: kernels compiled for 3 waves per SIMD (the autotuned choice on both precisions)
 template <class Work>
-void (*pick_adaptive(int prec, bool radio))(const Work, unsigned int*, int, int, BlockOut*) {
+void (*pick_adaptive(int prec, bool radio))(const Work, unsigned int*, int, int, int, BlockOut*) {
     if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, 3, Work> : k_persist<BH_PREC_FP64, false, 3, Work>;
     return radio ? k_persist<BH_PREC_MIXED, true, 3, Work> : k_persist<BH_PREC_MIXED, false, 3, Work>;
 }
@@ -682,8 +745,11 @@ PersistFn pick_persist(int prec, bool radio, int waves) {
 
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
 
-// Defaults: the autotuner's choice on the 1024^2 230 GHz frame for both precisions (MI355X).
-bh_batch_cfg batch_defaults() {
+// Defaults, measured on the 1024^2 230 GHz frame (MI355X). The mixed tracer's launch is bound by its
+// longest rays (up to 7x the mean step count): queueing them first (impact order) and raising the
+// priority of waves that hold rays older than 256 steps cut its frame time by a quarter. The FP64
+// tracer (at most 3x the mean) is throughput-bound and keeps row order without priorities.
+bh_batch_cfg batch_defaults(int prec) {
     bh_batch_cfg d{};
     d.block = 256;
     d.grid_per_cu = 4;
@@ -691,7 +757,40 @@ bh_batch_cfg batch_defaults() {
     d.refill_min = 32;
     d.use_graph = 1;
     d.waves = 3;
+    d.order = prec == BH_PREC_MIXED ? BH_ORDER_IMPACT : BH_ORDER_ROWS;
+    d.prio_age = prec == BH_PREC_MIXED ? 256 : -1;
     return d;
+}
+
+// Replaces the 0 ("default") fields of a launch configuration and checks the ranges.
+int resolve_cfg(bh_ctx* c, int prec, bh_batch_cfg* g) {
+    const bh_batch_cfg d = batch_defaults(prec);
+    if (g->block == 0) g->block = d.block;
+    if (g->grid_per_cu == 0) g->grid_per_cu = d.grid_per_cu;
+    if (g->interval == 0) g->interval = d.interval;
+    if (g->refill_min == 0) g->refill_min = d.refill_min;
+    if (g->order == 0) g->order = d.order;
+    if (g->prio_age == 0) g->prio_age = d.prio_age;
+    if (g->block != 64 && g->block != 128 && g->block != 256) return fail(c, BH_ERR_ARG, "block must be 64, 128 or 256");
+    if (g->grid_per_cu < 1 || g->grid_per_cu > 32) return fail(c, BH_ERR_ARG, "grid_per_cu must be in [1, 32]");
+    if (g->interval < 1 || g->interval > 4096) return fail(c, BH_ERR_ARG, "interval must be in [1, 4096]");
+    if (g->refill_min < 1 || g->refill_min > 64) return fail(c, BH_ERR_ARG, "refill_min must be in [1, 64]");
+    if (g->waves != 0 && g->waves != 3 && g->waves != 4) return fail(c, BH_ERR_ARG, "waves must be 0, 3 or 4");
+    if (g->order != BH_ORDER_ROWS && g->order != BH_ORDER_IMPACT) return fail(c, BH_ERR_ARG, "order must be 0, 1 or 2");
+    if (g->prio_age > (1 << 24)) return fail(c, BH_ERR_ARG, "prio_age must be at most 2^24");
+    return BH_OK;
+}
+
+// Enqueues the counting sort that fills each frame's tile order (frames already on the device;
+// hist holds n * ORDER_BUCKETS counters).
+void enqueue_tile_order(hipStream_t st, const bhp::BatchFrame* dframes, int n, unsigned int max_tiles,
+                        unsigned int* hist) {
+    const int nh = n * ORDER_BUCKETS;
+    hipLaunchKernelGGL(k_zero_u32, dim3((nh + 255) / 256), dim3(256), 0, st, hist, nh);
+    const dim3 og((max_tiles + 255) / 256, n);
+    hipLaunchKernelGGL(k_order_tiles<false>, og, dim3(256), 0, st, dframes, hist);
+    hipLaunchKernelGGL(k_order_scan, dim3(n), dim3(256), 0, st, hist);
+    hipLaunchKernelGGL(k_order_tiles<true>, og, dim3(256), 0, st, dframes, hist);
 }
 
 // Traces a batch with one persistent launch: [queue reset, k_persist, k_reduce], captured into a
@@ -703,17 +802,9 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     if (opts) o = *opts;
     int rc = check_opts(c, o);
     if (rc) return rc;
-    bh_batch_cfg cfg = cfg_in ? *cfg_in : (c->tuned_ok ? c->tuned : batch_defaults());
-    const bh_batch_cfg d = batch_defaults();
-    if (cfg.block == 0) cfg.block = d.block;
-    if (cfg.grid_per_cu == 0) cfg.grid_per_cu = d.grid_per_cu;
-    if (cfg.interval == 0) cfg.interval = d.interval;
-    if (cfg.refill_min == 0) cfg.refill_min = d.refill_min;
-    if (cfg.block != 64 && cfg.block != 128 && cfg.block != 256) return fail(c, BH_ERR_ARG, "block must be 64, 128 or 256");
-    if (cfg.grid_per_cu < 1 || cfg.grid_per_cu > 32) return fail(c, BH_ERR_ARG, "grid_per_cu must be in [1, 32]");
-    if (cfg.interval < 1 || cfg.interval > 4096) return fail(c, BH_ERR_ARG, "interval must be in [1, 4096]");
-    if (cfg.refill_min < 1 || cfg.refill_min > 64) return fail(c, BH_ERR_ARG, "refill_min must be in [1, 64]");
-    if (cfg.waves != 0 && cfg.waves != 3 && cfg.waves != 4) return fail(c, BH_ERR_ARG, "waves must be 0, 3 or 4");
+    bh_batch_cfg cfg = cfg_in ? *cfg_in : (c->tuned_ok ? c->tuned : batch_defaults(o.precision));
+    rc = resolve_cfg(c, o.precision, &cfg);
+    if (rc) return rc;
     const int mode = frames[0].scene.mode;
     long long total = 0;
     std::vector<bhp::BatchFrame> hb((size_t)n);
@@ -748,6 +839,23 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
         hb[(size_t)i].out = c->d_bout + off;
         off += (long long)frames[i].frame.W * frames[i].frame.H;
     }
+    // tile order (grid y = frame, so at most 65535 frames are sorted; larger batches keep row order)
+    const bool ordered = cfg.order == BH_ORDER_IMPACT && n <= 65535;
+    unsigned int max_tiles = 0;
+    if (ordered) {
+        rc = ensure(c, &c->d_order, &c->order_cap, (size_t)(total + 63));   // >= tiles of all frames
+        if (rc) return rc;
+        rc = ensure(c, &c->d_ohist, &c->ohist_cap, (size_t)n * ORDER_BUCKETS);
+        if (rc) return rc;
+        long long toff = 0;
+        for (int i = 0; i < n; ++i) {
+            const unsigned int nt = hb[(size_t)i].nentries >> 6;
+            hb[(size_t)i].order = c->d_order + toff;
+            toff += nt;
+            max_tiles = std::max(max_tiles, nt);
+        }
+    }
+    const int prio = cfg.prio_age > 0 ? cfg.prio_age : 0;
     const int grid = c->n_cu * cfg.grid_per_cu;
     rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof(bhp::BatchFrame) * (size_t)n);
     if (rc) return rc;
@@ -760,8 +868,9 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
+        if (ordered) enqueue_tile_order(c->st, dframes, n, max_tiles, c->d_ohist);
         hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), 0, c->st, bhp::BatchWork{dframes, n}, c->d_queue,
-                           cfg.interval, cfg.refill_min, c->d_blk);
+                           cfg.interval, cfg.refill_min, prio, c->d_blk);
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
         return hipGetLastError();
     };
@@ -774,7 +883,9 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
                                             (unsigned long long)(uintptr_t)c->d_blk,
                                             (unsigned long long)(uintptr_t)c->d_tot, (unsigned long long)n,
                                             (unsigned long long)grid, (unsigned long long)cfg.block,
-                                            (unsigned long long)cfg.interval, (unsigned long long)cfg.refill_min};
+                                            (unsigned long long)cfg.interval, (unsigned long long)cfg.refill_min,
+                                            (unsigned long long)ordered, (unsigned long long)max_tiles,
+                                            (unsigned long long)(uintptr_t)c->d_ohist, (unsigned long long)prio};
         for (unsigned long long v : parts) key = (key ^ v) * 1099511628211ull;
         if (!c->graph || c->graph_key != key) {
             if (c->graph) {
@@ -838,8 +949,12 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     if ((long long)tiles * 64 * spp >= (1ll << 31) ||
         (long long)npix * ((extra_max + ADAPT_CHUNK - 1) / ADAPT_CHUNK) * ADAPT_CHUNK >= (1ll << 31))
         return fail(c, BH_ERR_ARG, "frame too large for adaptive sampling (2^31 samples)");
-    bh_batch_cfg cfg = c->tuned_ok ? c->tuned : batch_defaults();
+    bh_batch_cfg cfg = c->tuned_ok ? c->tuned : batch_defaults(o.precision);
+    rc = resolve_cfg(c, o.precision, &cfg);
+    if (rc) return rc;
     const int grid = c->n_cu * cfg.grid_per_cu;
+    const int prio = cfg.prio_age > 0 ? cfg.prio_age : 0;
+    const bool ordered = cfg.order == BH_ORDER_IMPACT;
     bhp::BatchFrame B[2];
     std::memset(B, 0, sizeof B);
     B[0].P = make_params(c, f);
@@ -856,6 +971,16 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     B[0].nentries = (unsigned int)(tiles * 64 * spp);
     B[1] = B[0];
     const size_t n = (size_t)npix;
+    if (ordered) {
+        if ((rc = ensure(c, &c->d_order, &c->order_cap, (size_t)tiles)) ||
+            (rc = ensure(c, &c->d_ohist, &c->ohist_cap, (size_t)ORDER_BUCKETS)))
+            return rc;
+        B[0].order = c->d_order;
+        // the sort reads nentries / 64 tiles from its descriptor; pass 1 has spp entries per pixel, so
+        // the sort gets slot 1 (pass 2's, written again before pass 2) with one entry per pixel
+        B[1] = B[0];
+        B[1].nentries = (unsigned int)(tiles * 64);
+    }
     if ((rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof B)) || (rc = ensure(c, &c->d_queue, &c->queue_cap, 1)) ||
         (rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid)) ||
         (rc = ensure(c, &c->d_samp1, &c->samp1_cap, n * spp)) || (rc = ensure(c, &c->d_acc, &c->acc_cap, n)) ||
@@ -871,8 +996,10 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     HIPCK(c, hipMemcpyAsync(c->d_bframes, B, sizeof B, hipMemcpyHostToDevice, c->st));
     // pass 1
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
+    if (ordered) enqueue_tile_order(c->st, dB + 1, 1, (unsigned int)tiles, c->d_ohist);
     hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
-                       bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, c->d_blk);
+                       bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, prio,
+                       c->d_blk);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
     HIPCK(c, hipGetLastError());
     BlockOut tot1, tot2{};
@@ -902,12 +1029,13 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
             rc = ensure(c, &c->d_samp2, &c->samp2_cap, (size_t)total * ADAPT_CHUNK);
             if (rc) return rc;
             B[1].nentries = total * ADAPT_CHUNK;
+            B[1].order = nullptr;
             HIPCK(c, hipMemcpyAsync(c->d_bframes + sizeof(bhp::BatchFrame), &B[1], sizeof B[1], hipMemcpyHostToDevice,
                                     c->st));
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
             hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
                                bhp::ChunkWork{dB + 1, 1, spp, npix, c->d_offsets, c->d_samp2}, c->d_queue,
-                               cfg.interval, cfg.refill_min, c->d_blk);
+                               cfg.interval, cfg.refill_min, prio, c->d_blk);
             hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
             HIPCK(c, hipGetLastError());
             HIPCK(c, hipMemcpyAsync(&tot2, c->d_tot, sizeof tot2, hipMemcpyDeviceToHost, c->st));
@@ -1142,7 +1270,7 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
     if (!c || !best) return BH_ERR_ARG;
     HIPCK(c, hipSetDevice(c->device));
     // coordinate descent from the defaults: one parameter at a time over its candidate list, two passes
-    bh_batch_cfg cur = batch_defaults();
+    bh_batch_cfg cur = batch_defaults(opts ? opts->precision : BH_PREC_FP64);
     int count = 0;
     double cur_ms = 0.0;
     auto time_cfg = [&](const bh_batch_cfg& g, double* ms) -> int {
@@ -1153,13 +1281,15 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
         if (rc) return rc;
         *ms = s.ms_kernel;
         if (tried && count < max_tried) {
-            double* row = tried + 6 * (size_t)count;
+            double* row = tried + 8 * (size_t)count;
             row[0] = g.block;
             row[1] = g.grid_per_cu;
             row[2] = g.interval;
             row[3] = g.refill_min;
             row[4] = s.ms_kernel;
             row[5] = g.waves;
+            row[6] = g.order;
+            row[7] = g.prio_age;
         }
         ++count;
         return BH_OK;
@@ -1167,15 +1297,17 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
     int rc = time_cfg(cur, &cur_ms);
     if (rc) return rc;
     const int blocks[] = {64, 128, 256}, grids[] = {1, 2, 3, 4, 8}, intervals[] = {2, 4, 8, 16, 32, 64},
-              refills[] = {1, 8, 16, 32, 48}, waves[] = {0, 3, 4};
+              refills[] = {1, 8, 16, 32, 48}, waves[] = {0, 3, 4}, orders[] = {BH_ORDER_ROWS, BH_ORDER_IMPACT},
+              ages[] = {-1, 64, 128, 256, 512};
     for (int pass = 0; pass < 2; ++pass) {
-        for (int p = 0; p < 5; ++p) {
-            const int* vals = p == 0 ? waves : p == 1 ? blocks : p == 2 ? grids : p == 3 ? intervals : refills;
-            const int nv = p == 0 ? 3 : p == 1 ? 3 : p == 2 ? 5 : p == 3 ? 6 : 5;
+        for (int p = 0; p < 7; ++p) {
+            const int* vals = p == 0 ? waves : p == 1 ? blocks : p == 2 ? grids : p == 3 ? intervals
+                            : p == 4 ? refills : p == 5 ? orders : ages;
+            const int nv = p == 0 ? 3 : p == 1 ? 3 : p == 2 ? 5 : p == 3 ? 6 : p == 4 ? 5 : p == 5 ? 2 : 5;
             for (int i = 0; i < nv; ++i) {
                 bh_batch_cfg g = cur;
                 int* field = p == 0 ? &g.waves : p == 1 ? &g.block : p == 2 ? &g.grid_per_cu
-                           : p == 3 ? &g.interval : &g.refill_min;
+                           : p == 3 ? &g.interval : p == 4 ? &g.refill_min : p == 5 ? &g.order : &g.prio_age;
                 if (*field == vals[i]) continue;
                 *field = vals[i];
                 double ms;
@@ -1255,7 +1387,7 @@ void bh_destroy(bh_ctx* c) {
     void* bufs[] = {c->d_k2,     c->d_turb,   c->d_px,     c->d_rays,    c->d_blk,    c->d_tot,    c->d_ph,
                     c->d_fates,  c->d_mlp,    c->d_adam_m, c->d_adam_v,  c->d_stab,   c->d_sbad,   c->d_bframes,
                     c->d_queue,  c->d_bout,   c->d_samp1,  c->d_acc,     c->d_ystat,  c->d_extra,  c->d_chunks,
-                    c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2};
+                    c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2,  c->d_order,  c->d_ohist};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

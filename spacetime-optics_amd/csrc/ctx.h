@@ -57,6 +57,10 @@ struct bh_ctx {
     size_t bframes_cap = 0;               // bytes
     unsigned int* d_queue = nullptr;
     size_t queue_cap = 0;
+    unsigned int* d_order = nullptr;      // queue position -> tile, all frames of a launch
+    size_t order_cap = 0;
+    unsigned int* d_ohist = nullptr;      // per-frame bin counters of the tile sort
+    size_t ohist_cap = 0;
     float4* d_bout = nullptr;
     size_t bout_cap = 0;
     hipGraphExec_t graph = nullptr;

@@ -1,14 +1,18 @@
 /* persist.h - persistent multi-frame tracer (device code, included by bhrt.hip).
 
    Workgroups stay resident and pull rays from per-frame work queues (one atomic counter per frame;
-   queue entries are the rays of 8 x 8 pixel tiles in tile order, so the rays a wave takes at once are
-   image neighbours). Every `interval` integration steps a wave retires finished lanes (one ballot),
+   queue entries are the rays of 8 x 8 pixel tiles, so the rays a wave takes at once are image
+   neighbours). The tiles of a frame are queued in row order or in a given order (the launcher sorts
+   them by the impact parameter of their centre ray, so the rays that orbit the hole longest start
+   first). Every `interval` integration steps a wave retires finished lanes (one ballot),
    writes their pixels, and refills the free lanes with new rays when at least `refill_min` lanes are
    free: lane 0 claims a contiguous range with one atomic add and each free lane takes the entry at
    its rank among the free lanes. A wave refills only from its current frame and moves to the next
    frame once its lanes are drained, so the frame parameters stay wave-uniform (scalar loads).
    Every wave exits when all frames are exhausted and its lanes are idle; each ray runs at most
-   max_steps steps, so the grid always drains. What an entry means (which pixel and sample, where
+   max_steps steps, so the grid always drains. With prio_age > 0 a wave that holds a ray older than
+   prio_age steps raises its issue priority, so the longest rays run at their own latency instead
+   of sharing the SIMD evenly with younger waves. What an entry means (which pixel and sample, where
    the finished ray is written) is a Work policy: frame batches, and the two passes of adaptive
    sampling. */
 #pragma once
@@ -26,7 +30,12 @@ struct BatchFrame {
     unsigned int nentries; // queue entries of the frame
     int tiles_x;
     float4* out;           // W*H pixels (frame batches)
+    unsigned int* order;   // queue position -> tile (null: row order)
 };
+
+__device__ __forceinline__ unsigned int queued_tile(const BatchFrame& F, unsigned int pos) {
+    return F.order ? F.order[pos] : pos;
+}
 
 __device__ __forceinline__ void tile_pixel(int tiles_x, unsigned int tile, unsigned int within, int* px, int* py) {
     *px = (int)(tile % (unsigned)tiles_x) * 8 + (int)(within & 7u);
@@ -41,13 +50,13 @@ struct BatchWork {
     const BatchFrame* frames;
     int n;
     __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
-        tile_pixel(frames[f].tiles_x, e >> 6, e & 63u, px, py);
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e >> 6), e & 63u, px, py);
         *s = 0;
         return *px < frames[f].P.w && *py < frames[f].P.h;
     }
     __device__ __forceinline__ void finish(int f, unsigned int e, const RayResult& r) const {
         int px, py;
-        tile_pixel(frames[f].tiles_x, e >> 6, e & 63u, &px, &py);
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e >> 6), e & 63u, &px, &py);
         frames[f].out[(size_t)py * frames[f].P.w + px] =
             make_float4((float)r.I[0], (float)r.I[1], (float)r.I[2], (float)r.I[3]);
     }
@@ -62,14 +71,14 @@ struct SampleWork {
     double4* out;
     __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
         const unsigned int per_tile = 64u * (unsigned)spp, rem = e % per_tile;
-        tile_pixel(frames[f].tiles_x, e / per_tile, rem & 63u, px, py);
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e / per_tile), rem & 63u, px, py);
         *s = (int)(rem >> 6);
         return *px < frames[f].P.w && *py < frames[f].P.h;
     }
     __device__ __forceinline__ void finish(int f, unsigned int e, const RayResult& r) const {
         const unsigned int per_tile = 64u * (unsigned)spp, rem = e % per_tile;
         int px, py;
-        tile_pixel(frames[f].tiles_x, e / per_tile, rem & 63u, &px, &py);
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e / per_tile), rem & 63u, &px, &py);
         out[((size_t)py * frames[f].P.w + px) * spp + (rem >> 6)] = make_double4(r.I[0], r.I[1], r.I[2], r.I[3]);
     }
 };
@@ -140,7 +149,8 @@ struct Tracer<BH_PREC_MIXED, RADIO> {
 
 template <int PREC, bool RADIO, class Work>
 __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __restrict__ queue, int interval,
-                                             int refill_min, double* flux_out, unsigned long long* cnt_out) {
+                                             int refill_min, int prio_age, double* flux_out,
+                                             unsigned long long* cnt_out) {
     using T = Tracer<PREC, RADIO>;
     const BatchFrame* __restrict__ frames = Wk.frames;
     const int nframes = Wk.n;
@@ -189,6 +199,10 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
             }
         }
         if (__ballot(active) == 0ull && cur >= nframes) break;
+        if (prio_age > 0) {   // wave-uniform: any lane with an old ray lifts the wave
+            if (__ballot(active && R.it >= prio_age)) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         const int cf = __builtin_amdgcn_readfirstlane(cur < nframes ? cur : nframes - 1);
         // frame parameters through the constant address space: wave-uniform scalar loads
         using CFrame = const __attribute__((address_space(4))) BatchFrame;
@@ -213,6 +227,7 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
             }
         }
     }
+    if (prio_age > 0) __builtin_amdgcn_s_setprio(0);
     *flux_out = flux;
     cnt_out[0] = c0;
     cnt_out[1] = c1;

@@ -208,6 +208,45 @@ __device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const Em
     *ln_a = *ln_j - ln_bnu;
 }
 
+// Turbulence density factor in FP32: the grid, cell-centred trilinear weights, edge clamps and
+// time-slice interpolation of bhd::turb_factor, with the azimuth reduced to [0, 2 pi) in FP64 before
+// the FP32 sine and cosine (an orbiting photon's phi grows by 2 pi per turn).
+__device__ __forceinline__ float grid_trilinear_f32(const TraceParams& P, const float* __restrict__ G, float px,
+                                                    float py, float pz) {
+    const int n = P.turb_n;
+    const float half = (float)P.turb_half, inv = (float)P.turb_inv_dx, hi = (float)(n - 1);
+    const float gx = fminf(fmaxf((px + half) * inv - 0.5f, 0.f), hi);
+    const float gy = fminf(fmaxf((py + half) * inv - 0.5f, 0.f), hi);
+    const float gz = fminf(fmaxf((pz + half) * inv - 0.5f, 0.f), hi);
+    const int ix = min((int)gx, n - 2), iy = min((int)gy, n - 2), iz = min((int)gz, n - 2);
+    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    const size_t sy = (size_t)n, sz = (size_t)n * n;
+    const float* b = G + ((size_t)iz * sz + (size_t)iy * sy + (size_t)ix);
+    const float v000 = b[0], v100 = b[1], v010 = b[sy], v110 = b[sy + 1];
+    const float v001 = b[sz], v101 = b[sz + 1], v011 = b[sz + sy], v111 = b[sz + sy + 1];
+    const float c00 = v000 + (v100 - v000) * fx, c10 = v010 + (v110 - v010) * fx;
+    const float c01 = v001 + (v101 - v001) * fx, c11 = v011 + (v111 - v011) * fx;
+    const float c0 = c00 + (c10 - c00) * fy, c1 = c01 + (c11 - c01) * fy;
+    return c0 + (c1 - c0) * fz;
+}
+
+__device__ __forceinline__ float turb_factor_f32(const TraceParams& P, float r, float u, double phid, double td) {
+    const float R = r * __builtin_sqrtf(fmaxf(1.f - u * u, 1e-12f));
+    const double two_pi = 6.283185307179586;
+    const float ph = (float)(phid - two_pi * floor(phid * (1.0 / two_pi)));
+    float sp, cp;
+    __sincosf(ph, &sp, &cp);
+    const float px = R * cp, py = R * sp, pz = r * u;
+    if (P.turb_nt <= 1) return grid_trilinear_f32(P, P.turb, px, py, pz);
+    const float st = fminf(fmaxf((float)((td - P.turb_t0) * P.turb_inv_dt), 0.f), (float)(P.turb_nt - 1));
+    const int k = min((int)st, P.turb_nt - 2);
+    const float f = st - (float)k;
+    const size_t slice = (size_t)P.turb_n * P.turb_n * P.turb_n;
+    const float A = grid_trilinear_f32(P, P.turb + (size_t)k * slice, px, py, pz);
+    const float B = grid_trilinear_f32(P, P.turb + (size_t)(k + 1) * slice, px, py, pz);
+    return A + (B - A) * f;
+}
+
 // Geometry + plasma of one flow sample; returns false when nothing is emitted or absorbed.
 struct FlowSample {
     float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl, ln_bnu;
@@ -225,7 +264,7 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
     const float lr = lnf(r);
     float ln_ne = E.ln_n0 - E.n_index * lr + lnvert;
-    if (P.turb_on) ln_ne += lnf((float)bhd::turb_factor(P, rd, ud, phid, td));
+    if (P.turb_on) ln_ne += lnf(turb_factor_f32(P, r, u, phid, td));
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
     if (!(soft > 0.f)) return false;

@@ -61,11 +61,13 @@ def test_batch_single_frame_every_config(c2, prec):
     cfg, t = c2
     o = t.opts(precision=prec)
     want, flux, cnt = render_each(t, [(cfg["S"], cfg["F"])], o)
-    for block, gpc, interval, refill, graph in [(256, 2, 8, 16, 1), (256, 2, 8, 16, 0), (64, 1, 1, 1, 0),
-                                                (128, 4, 32, 48, 1), (256, 8, 3, 64, 1)]:
+    for block, gpc, interval, refill, graph, order, age in [
+            (256, 2, 8, 16, 1, bh.ORDER_ROWS, -1), (256, 2, 8, 16, 0, bh.ORDER_IMPACT, -1),
+            (64, 1, 1, 1, 0, bh.ORDER_ROWS, 64), (128, 4, 32, 48, 1, bh.ORDER_IMPACT, 128),
+            (256, 8, 3, 64, 1, bh.ORDER_IMPACT, 1), (256, 4, 32, 32, 1, 0, 0)]:
         got, st = t.trace_batch([(cfg["S"], cfg["F"])], o, block=block, grid_per_cu=gpc, interval=interval,
-                                refill_min=refill, use_graph=graph)
-        assert_same(got, want, f"prec {prec} cfg {(block, gpc, interval, refill, graph)}")
+                                refill_min=refill, use_graph=graph, order=order, prio_age=age)
+        assert_same(got, want, f"prec {prec} cfg {(block, gpc, interval, refill, graph, order, age)}")
         assert np.array_equal(stats_vec(st), cnt)
         assert st.samples == 256 * 256
         assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
@@ -83,10 +85,11 @@ def test_batch_mixed_sizes_and_spins(c2, prec):
     o = t.opts(precision=prec)
     want, flux, cnt = render_each(t, frames, o)
     t.update_scene(cfg["S"])
-    got, st = t.trace_batch(frames, o)
-    assert_same(got, want, f"batch prec {prec}")
-    assert np.array_equal(stats_vec(st), cnt)
-    assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
+    for order, age in [(0, 0), (bh.ORDER_ROWS, -1), (bh.ORDER_IMPACT, 32)]:
+        got, st = t.trace_batch(frames, o, order=order, prio_age=age)
+        assert_same(got, want, f"batch prec {prec} order {order} prio_age {age}")
+        assert np.array_equal(stats_vec(st), cnt)
+        assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
 
 
 def test_batch_turbulence_time_slices():
@@ -158,6 +161,10 @@ def test_batch_errors(c2):
         t.trace_batch([])
     with pytest.raises(bh.BhError, match="MLP emissivity"):
         t.trace_batch([(cfg["S"], cfg["F"])], t.opts(precision=bh.PREC_FP64, emissivity=bh.EMIS_MLP))
+    with pytest.raises(bh.BhError, match="order must be"):
+        t.trace_batch([(cfg["S"], cfg["F"])], order=3)
+    with pytest.raises(bh.BhError, match="prio_age must be"):
+        t.trace_batch([(cfg["S"], cfg["F"])], prio_age=(1 << 24) + 1)
 
 
 @pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
