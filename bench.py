@@ -11,7 +11,8 @@ restated in C++) on a bounded random pixel sample of the same frame, over all ho
 
 Modes: fp64 / mixed / mixed_mlp run one launch per frame (bh_render_resident); p_* run the same
 tracers through the persistent batched kernel (bh_trace_batch, graph replay); p_mixed_f32far is
-the mixed tracer with FP32 stages outside the flow. `value` is the
+the mixed tracer with FP32 stages outside the flow, p_mixed_fullflow the mixed tracer with emission,
+absorption and the transfer step cap over the whole flow (default: inside the emission region). `value` is the
 fastest mode whose frame is within 0.1% flux and 1% median per-pixel error of the FP64 frame.
 """
 import argparse
@@ -152,14 +153,15 @@ def cpu_baseline(S, k2, l0, dl, F, n_pix, seed=11):
                       f"mean {rays['steps'].mean():.1f} DP54 steps/ray"}
 
 
-MODES = {   # name: (precision, emissivity, persistent, FP32 far-field stages)
-    "fp64": (bh.PREC_FP64, bh.EMIS_EXACT, False, 0),
-    "mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, False, 0),
-    "mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, False, 0),
-    "p_fp64": (bh.PREC_FP64, bh.EMIS_EXACT, True, 0),
-    "p_mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, True, 0),
-    "p_mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, True, 0),
-    "p_mixed_f32far": (bh.PREC_MIXED, bh.EMIS_EXACT, True, 1),
+MODES = {   # name: (precision, emissivity, persistent, FP32 far-field stages, emission over the whole flow)
+    "fp64": (bh.PREC_FP64, bh.EMIS_EXACT, False, 0, 0),
+    "mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, False, 0, 0),
+    "mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, False, 0, 0),
+    "p_fp64": (bh.PREC_FP64, bh.EMIS_EXACT, True, 0, 0),
+    "p_mixed": (bh.PREC_MIXED, bh.EMIS_EXACT, True, 0, 0),
+    "p_mixed_mlp": (bh.PREC_MIXED, bh.EMIS_MLP, True, 0, 0),
+    "p_mixed_f32far": (bh.PREC_MIXED, bh.EMIS_EXACT, True, 1, 0),
+    "p_mixed_fullflow": (bh.PREC_MIXED, bh.EMIS_EXACT, True, 0, 1),
 }
 FLUX_GATE, PIXEL_GATE = 1e-3, 1e-2   # a mode counts only below 0.1% flux and 1% median per-pixel error
 
@@ -217,8 +219,8 @@ def main():
 
     results = {}
     for m in modes:
-        prec, emis, persistent, f32 = MODES[m]
-        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32)
+        prec, emis, persistent, f32, full = MODES[m]
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full)
         try:
             cfg = None
             if persistent and not args.no_autotune:
@@ -236,8 +238,8 @@ def main():
     # accuracy of every mode against the FP64 tracer (bit-identical to the oracle) on this frame
     ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1))
     for m, r in results.items():
-        prec, emis, persistent, f32 = MODES[m]
-        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32)
+        prec, emis, persistent, f32, full = MODES[m]
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full)
         cfg = None
         if f"autotune_{m}" in setup:
             cfg = bh.BatchCfg(**setup[f"autotune_{m}"]["best"])

@@ -90,14 +90,19 @@ typedef struct bh_trace_opts {
                          radio: 1e-4 * (pix * r) / 0.04 M, the pixel's size at the hole, clamped to
                          [max(1e-6, tol), 1e-4]) */
     double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
-    double r_emit;    /* mixed, radio: the scene tolerance below max(r_emit, r_near), max(1e-6, tol) from
-                         there to the flow's edge, tol_far beyond (0 -> the radius outside which the
-                         scene's flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
-                         emissivity) */
+    double r_emit;    /* mixed, radio: the emission region is r < max(r_emit, r_near): scene tolerance,
+                         emission, absorption and the transfer step cap there; tol_mid from there to
+                         the flow's edge, tol_far beyond (0 -> the radius outside which the scene's
+                         flow model, in its mid-plane, emits less than 1e-6 of its r^2-weighted
+                         emissivity and has a radial optical depth below 1e-7) */
     int far_fp32;     /* mixed: 0 -> FP64 stages everywhere (default: on MI355X FP32 arithmetic without
                          packed FP32 runs at the FP64 rate, and a per-lane precision switch costs
                          both stage codes per wave); 1 -> FP32 stages in the far field (radio: outside
                          the flow; vacuum: outside r_near), same tolerances */
+    int full_flow;    /* mixed, radio: 0 (default) -> the flow outside the emission region is traced as
+                         vacuum with tol_mid = max(max(1e-6, tol), tol_far / 10); 1 -> emission,
+                         absorption and the step cap over the whole flow as the FP64 tracer, with
+                         tol_mid = max(1e-6, tol) */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */
@@ -264,6 +269,38 @@ int bh_mlp_grad(bh_ctx* ctx, const float* params, const float* norm, const float
    points (|d ln f| > judge_tol, 0 -> 2e-3, fails it). The mixed tracer with BH_EMIS_MLP reads this
    table and uses the exact formula in failed cells and outside the box. */
 int bh_mlp_bake(bh_ctx* ctx, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_report* report);
+
+/* ---- Radio post-processing and display (the radio half of Renderer::render's post step,
+   render.cu:742-753,756-792, and of Renderer::tonemap, render.cu:811-846, without the HUD overlay).
+   Frames are linear HDR float4 pixels (I, I_n1, I_n2, I_n>=3) as bh_render writes them. ---- */
+typedef struct bh_post_params {     /* PostParams (renderer.hpp:31-35), radio fields */
+    double blur_sigma_px;           /* Gaussian beam sigma in pixels (applied when > 0.5) */
+    float hide_direct;              /* fraction of the direct (n = 0) light left out of the statistics */
+} bh_post_params;
+
+/* Beam blur in place on px (W*H*4 floats, host memory), then the frame statistics of the displayed
+   intensity I - hide_direct * max(I - I_n1 - I_n2 - I_n>=3, 0): lum_max, and lum_percentile = the
+   `percentile` quantile of the pixels above 1e-5 of the maximum (Renderer::render,
+   render.cu:756-792). Either output pointer may be NULL. */
+int bh_post_radio(bh_ctx* ctx, float* px, int W, int H, const bh_post_params* params, double percentile,
+                  double* lum_percentile, double* lum_max);
+
+typedef struct bh_tone_params {     /* ToneParams (renderer.hpp:48-59), radio fields */
+    int colormap;                   /* 0 afmhot (EHT), 1 inferno */
+    float exposure;                 /* multiplies the displayed intensity */
+    float gamma;                    /* display gamma */
+    float overlay;                  /* photon-ring order tint, 0..1 */
+    float hide_direct;              /* fraction of the n = 0 light removed */
+    float fade;                     /* fade to black, 1 = none */
+    float vignette;
+    float grain;                    /* film grain amplitude (PCG noise per pixel) */
+    unsigned int seed;
+} bh_tone_params;
+
+/* Radio tone map of px (W*H*4 floats, host memory) to rgb8 (W*H*3 bytes, row-major from the top
+   left): colormap of (exposure * displayed intensity)^gamma, highlights burned to white, optional
+   order tint, vignette, grain and dithered 8-bit rounding (k_tonemap, render.cu:360-407). */
+int bh_tonemap(bh_ctx* ctx, const float* px, int W, int H, const bh_tone_params* params, unsigned char* rgb8);
 
 /* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context
    needed; device 0). fn: 0 exp, 1 log, 2 expm1, 3 cbrt, 4 x^-0.2, 5 sin, 6 cos, 7 atan2(x, y),

@@ -134,6 +134,14 @@ void or_math_eval(int fn, const double* x, const double* y, double* out, int n);
 // analytic -, analytic +) and the unprojected radial-constraint drift.
 void or_selftest(double rows[8][6], double* drift, int* drift_steps);
 
+/* Radio post step and tone map (post.cpp; render.cu:244-407,742-791): beam blur in place on px
+   (W*H*4 floats), the displayed-intensity statistics, the radio tone map to W*H*3 bytes. */
+void or_post_blur(float* px, int W, int H, double sigma);
+void or_frame_stats(const float* px, int W, int H, float hide_direct, double percentile, double* lum_percentile,
+                    double* lum_max);
+void or_tonemap_radio(const float* px, int W, int H, int colormap, float exposure, float gamma, float overlay,
+                      float hide_direct, float fade, float vignette, float grain, uint32_t seed, uint8_t* rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,10 @@ class Oracle:
         L.or_turb_factor.restype = C.c_double
         L.or_turb_factor.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_flow_step.argtypes = [C.POINTER(OrScene)] + [C.c_double] * 7 + [_D, _D]
+        L.or_post_blur.argtypes = [_F, C.c_int, C.c_int, C.c_double]
+        L.or_frame_stats.argtypes = [_F, C.c_int, C.c_int, C.c_float, C.c_double, _D, _D]
+        L.or_tonemap_radio.argtypes = ([_F, C.c_int, C.c_int, C.c_int] + [C.c_float] * 7 +
+                                       [C.c_uint32, C.POINTER(C.c_uint8)])
         L.or_escape_angles.argtypes = [_D, _D, _D, _D]
         L.or_trace_ray.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.c_double, C.c_double, C.POINTER(OrRay)]
         L.or_trace_pixels.restype = C.c_int
@@ -226,6 +230,30 @@ class Oracle:
                                             nthreads, img.ctypes.data_as(_D),
                                             ns.ctypes.data_as(C.POINTER(C.c_uint32)))
         return img, ns, total
+
+    # --- radio post step and tone map (post.cpp) -----------------------------------------------
+    def post_blur(self, img, sigma):
+        """Beam blur of an H x W x 4 float32 image (a new array)."""
+        out = np.ascontiguousarray(img, dtype=np.float32).copy()
+        H, W = out.shape[:2]
+        self.lib.or_post_blur(out.ctypes.data_as(_F), W, H, sigma)
+        return out
+
+    def frame_stats(self, img, hide_direct=0.0, percentile=0.995):
+        px = np.ascontiguousarray(img, dtype=np.float32)
+        H, W = px.shape[:2]
+        lp, lm = C.c_double(), C.c_double()
+        self.lib.or_frame_stats(px.ctypes.data_as(_F), W, H, hide_direct, percentile, C.byref(lp), C.byref(lm))
+        return lp.value, lm.value
+
+    def tonemap_radio(self, img, colormap=0, exposure=1.0, gamma=1.0, overlay=0.0, hide_direct=0.0, fade=1.0,
+                      vignette=0.0, grain=0.0, seed=0):
+        px = np.ascontiguousarray(img, dtype=np.float32)
+        H, W = px.shape[:2]
+        rgb = np.zeros((H, W, 3), dtype=np.uint8)
+        self.lib.or_tonemap_radio(px.ctypes.data_as(_F), W, H, colormap, exposure, gamma, overlay, hide_direct, fade,
+                                  vignette, grain, seed, rgb.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return rgb
 
     def flow_step(self, S, r, u, phi, t, lam, Ecam, h):
         dI = C.c_double()

@@ -24,7 +24,7 @@ EXPORTS = [
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
-    "bh_trace_batch", "bh_autotune", "bh_render_adaptive",
+    "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap",
 ]
 MLP_NPARAM = 8706
 MLP_NNORM = 10   # input centres[3], half-widths[3], output means[2], scales[2]
@@ -64,7 +64,8 @@ class Settings(C.Structure):
 
 class TraceOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
-                ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int)]
+                ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int),
+                ("full_flow", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -88,6 +89,16 @@ class BatchCfg(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PostParams(C.Structure):
+    _fields_ = [("blur_sigma_px", C.c_double), ("hide_direct", C.c_float)]
+
+
+class ToneParams(C.Structure):
+    _fields_ = [("colormap", C.c_int), ("exposure", C.c_float), ("gamma", C.c_float), ("overlay", C.c_float),
+                ("hide_direct", C.c_float), ("fade", C.c_float), ("vignette", C.c_float), ("grain", C.c_float),
+                ("seed", C.c_uint32)]
 
 
 class MlpTrainCfg(C.Structure):
@@ -173,6 +184,10 @@ def load():
     L.bh_autotune.restype = C.c_int
     L.bh_autotune.argtypes = [C.c_void_p, C.POINTER(BatchFrame), C.c_int, C.POINTER(TraceOpts), C.POINTER(BatchCfg),
                               _D, _D, C.c_int, C.POINTER(C.c_int)]
+    L.bh_post_radio.restype = C.c_int
+    L.bh_post_radio.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(PostParams), C.c_double, _D, _D]
+    L.bh_tonemap.restype = C.c_int
+    L.bh_tonemap.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(ToneParams), C.POINTER(C.c_uint8)]
     L.bh_math_eval.restype = C.c_int
     L.bh_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
     L.bh_destroy.argtypes = [C.c_void_p]
@@ -396,8 +411,8 @@ class Tracer:
 
     @staticmethod
     def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0, r_emit=0.0,
-             far_fp32=0):
-        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32)
+             far_fp32=0, full_flow=0):
+        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)
@@ -444,6 +459,29 @@ class Tracer:
         self._check(self.L.bh_vacuum_fates(self.ctx, ph.ctypes.data_as(_D), len(ph), tol, r_escape, max_steps,
                                            fates.ctypes.data_as(C.POINTER(C.c_int))), "bh_vacuum_fates")
         return fates
+
+    # --- radio post step and display ---------------------------------------------------------
+    def post_radio(self, img, blur_sigma_px=0.0, hide_direct=0.0, percentile=0.995):
+        """Beam blur of an H x W x 4 float32 frame (returns a new array) and its display statistics
+        (lum_percentile, lum_max)."""
+        out = np.ascontiguousarray(img, dtype=np.float32).copy()
+        H, W = out.shape[:2]
+        pp = PostParams(blur_sigma_px, hide_direct)
+        lp, lm = C.c_double(), C.c_double()
+        self._check(self.L.bh_post_radio(self.ctx, out.ctypes.data_as(_F), W, H, C.byref(pp), percentile, C.byref(lp),
+                                         C.byref(lm)), "bh_post_radio")
+        return out, lp.value, lm.value
+
+    def tonemap(self, img, colormap=0, exposure=1.0, gamma=1.0, overlay=0.0, hide_direct=0.0, fade=1.0, vignette=0.0,
+                grain=0.0, seed=0):
+        """Radio tone map of an H x W x 4 float32 frame to H x W x 3 bytes."""
+        px = np.ascontiguousarray(img, dtype=np.float32)
+        H, W = px.shape[:2]
+        rgb = np.zeros((H, W, 3), dtype=np.uint8)
+        tp = ToneParams(colormap, exposure, gamma, overlay, hide_direct, fade, vignette, grain, seed)
+        self._check(self.L.bh_tonemap(self.ctx, px.ctypes.data_as(_F), W, H, C.byref(tp),
+                                      rgb.ctypes.data_as(C.POINTER(C.c_uint8))), "bh_tonemap")
+        return rgb
 
     # --- persistent batches -------------------------------------------------------------------
     @staticmethod

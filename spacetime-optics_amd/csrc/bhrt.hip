@@ -556,14 +556,16 @@ constexpr double C_C = 2.99792458e10, C_H = 6.62607015e-27, C_ME = 9.1093837e-28
    synchrotron emissivity (emission.cuh:130-170 at nu_obs, in the mid-plane, sin theta_B = 1, no
    Doppler factor, no turbulence) weighted by r^2 (a face-on annulus of height ~ h_r r), integrated
    outward from the horizon on a fine radial grid. Host-side, once per launch. */
-double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2_l0, float k2_dl, double frac) {
+double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2_l0, float k2_dl, double frac,
+                       double tau_max) {
     const int n = 2000;
     const double r0 = S.rh * 1.001, r1 = S.r_flow_max;
     if (!(r1 > r0) || k2.size() < 2) return r1;
-    std::vector<double> w(n);
+    std::vector<double> w(n), alpha(n);
     double tot = 0.0;
+    const double dr = (r1 - r0) / n;
     for (int i = 0; i < n; ++i) {
-        const double r = r0 + (r1 - r0) * (i + 0.5) / n;
+        const double r = r0 + dr * (i + 0.5);
         const double ne = S.n0 * std::pow(r, -S.n_index) * (1.0 - std::exp(-(r - S.rh) * (r - S.rh) / 0.25));
         const double th = S.theta0 * std::pow(r, -S.theta_index);
         const double B = std::sqrt(8.0 * C_PI * ne * C_MP * C_C * C_C / (6.0 * r * S.beta_p));
@@ -574,15 +576,23 @@ double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2
         const int k = (int)f;
         const double lk2 = k2[k] + (k2[k + 1] - k2[k]) * (f - k);
         const double brk = std::sqrt(X) + 1.887748625363387 * std::sqrt(x3);
-        const double j = ne * nus * std::exp(-lk2) * brk * brk * std::exp(-x3);
+        // j_nu in cgs (emission.cuh:165-166) and alpha_nu = j_nu / B_nu(T_e) (Kirchhoff)
+        const double j = ne * std::sqrt(2.0) * C_PI * C_QE * C_QE * nus / (3.0 * C_C) * std::exp(-lk2) * brk * brk *
+                         std::exp(-x3);
+        const double bnu = 2.0 * C_H * S.nu_obs * S.nu_obs * S.nu_obs / (C_C * C_C) /
+                           std::expm1(C_H * S.nu_obs / (C_ME * C_C * C_C * th));
         w[i] = std::isfinite(j) ? j * r * r : 0.0;
+        alpha[i] = std::isfinite(j) && bnu > 0.0 ? j / bnu : 0.0;
         tot += w[i];
     }
     if (!(tot > 0.0)) return r1;
-    double out = 0.0;
+    // outermost radius where the emission outside still exceeds frac of the total, or the mid-plane
+    // radial optical depth from there to the flow's edge exceeds tau_max
+    double out = 0.0, tau = 0.0;
     for (int i = n - 1; i >= 0; --i) {
         out += w[i];
-        if (out > frac * tot) return r0 + (r1 - r0) * (i + 1.0) / n;
+        tau += alpha[i] * dr * S.rg_cm;
+        if (out > frac * tot || tau > tau_max) return r0 + dr * (i + 1.0);
     }
     return r0;
 }
@@ -612,20 +622,24 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
         double r_emit = o.r_emit;
         if (!(r_emit > 0)) {
             // cached per flow model: a batch of frames usually shares a few scenes
-            const double key[9] = {S.n0, S.n_index, S.theta0, S.theta_index, S.beta_p, S.nu_obs, S.rh, S.r_flow_max,
-                                   (double)c->k2_l0};
+            const double key[10] = {S.n0, S.n_index, S.theta0, S.theta_index, S.beta_p, S.nu_obs, S.rh, S.r_flow_max,
+                                    (double)c->k2_l0, S.rg_cm};
             for (const auto& e : c->emit_cache)
                 if (std::memcmp(e.first.data(), key, sizeof key) == 0) r_emit = e.second;
             if (!(r_emit > 0)) {
-                r_emit = emission_radius(S, c->k2_host, c->k2_l0, c->k2_dl, 1e-6);
+                r_emit = emission_radius(S, c->k2_host, c->k2_l0, c->k2_dl, 1e-6, 1e-7);
                 if (c->emit_cache.size() >= 64) c->emit_cache.clear();
-                std::array<double, 9> k;
+                std::array<double, 10> k;
                 std::memcpy(k.data(), key, sizeof key);
                 c->emit_cache.emplace_back(k, r_emit);
             }
         }
         M.r_far = std::max(S.r_flow_max, r_near);
         M.r_tol = std::max(std::min(r_emit, S.r_flow_max), r_near);
+        M.r_cut = o.full_flow ? S.r_flow_max : M.r_tol;
+        // outside the emission region the steps no longer follow the transfer step cap: a tolerance
+        // between the scene's and the far field's keeps the entry point into the region accurate
+        if (!o.full_flow) M.tol_mid = std::max(M.tol_mid, M.tol_far * 0.1);
     }
     M.r_f32 = o.far_fp32 ? M.r_far : 1e300;
     bhm32::EmisParams& E = M.E;
@@ -1387,7 +1401,8 @@ void bh_destroy(bh_ctx* c) {
     void* bufs[] = {c->d_k2,     c->d_turb,   c->d_px,     c->d_rays,    c->d_blk,    c->d_tot,    c->d_ph,
                     c->d_fates,  c->d_mlp,    c->d_adam_m, c->d_adam_v,  c->d_stab,   c->d_sbad,   c->d_bframes,
                     c->d_queue,  c->d_bout,   c->d_samp1,  c->d_acc,     c->d_ystat,  c->d_extra,  c->d_chunks,
-                    c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2,  c->d_order,  c->d_ohist};
+                    c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2,  c->d_order,  c->d_ohist,
+                    c->d_post_lev, c->d_post_tmp, c->d_post_wt, c->d_post_rgb};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

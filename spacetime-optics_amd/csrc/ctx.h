@@ -26,7 +26,7 @@ struct bh_ctx {
     bh_settings rs{};
     float* d_k2 = nullptr;
     std::vector<float> k2_host;   // host copy of the K2 table (emission-radius estimate)
-    std::vector<std::pair<std::array<double, 9>, double>> emit_cache;   // flow model -> emission radius
+    std::vector<std::pair<std::array<double, 10>, double>> emit_cache;   // flow model -> emission radius
     int k2_n = 0;
     float k2_l0 = 0, k2_dl = 0;
     float* d_turb = nullptr;
@@ -61,6 +61,15 @@ struct bh_ctx {
     size_t order_cap = 0;
     unsigned int* d_ohist = nullptr;      // per-frame bin counters of the tile sort
     size_t ohist_cap = 0;
+    // radio post: pyramid levels, scratch level / tone-map input, Gaussian taps, 8-bit output (bytes)
+    void* d_post_lev = nullptr;
+    size_t post_lev_cap = 0;
+    void* d_post_tmp = nullptr;
+    size_t post_tmp_cap = 0;
+    void* d_post_wt = nullptr;
+    size_t post_wt_cap = 0;
+    void* d_post_rgb = nullptr;
+    size_t post_rgb_cap = 0;
     float4* d_bout = nullptr;
     size_t bout_cap = 0;
     hipGraphExec_t graph = nullptr;

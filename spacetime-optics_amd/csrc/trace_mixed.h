@@ -2,12 +2,15 @@
 
    Policy (chosen from CPU sweeps against the FP64 oracle on the 1024^2 230 GHz frame and from
    timings on MI355X):
-   - Tolerance regions: the scene tolerance inside r_tol (radio: the radius holding all but 1e-6 of
-     the flow's emission; vacuum: r_near, near the hole) - there the reference's steps are
-     tolerance-limited and the radiative-transfer quadrature depends on that step sequence;
-     tol_mid from r_tol to r_far (the flow's edge in radio mode), where the emission is negligible
-     and the steps are set by the transfer step cap; tol_far beyond (radio: scaled with the
-     pixel's size, 1e-4 on 0.04 M pixels; vacuum: 1e-6).
+   - Tolerance regions: the scene tolerance inside r_tol (radio: the emission region, the radius
+     holding all but 1e-6 of the flow's emission with a negligible optical depth outside; vacuum:
+     r_near, near the hole) - there the reference's steps are tolerance-limited and the
+     radiative-transfer quadrature depends on that step sequence; tol_mid from r_tol to r_far (the
+     flow's edge in radio mode); tol_far beyond (radio: scaled with the pixel's size, 1e-4 on
+     0.04 M pixels; vacuum: 1e-6).
+   - Emission region: emission, absorption and the transfer step cap are evaluated below r_cut =
+     r_tol; the rest of the flow is traced as vacuum with tol_mid = tol_far / 10 (or, with
+     full_flow, the whole flow as the FP64 tracer with tol_mid = 1e-6).
    - Stage precision: FP64 stages evaluated with fused multiply-adds and Newton-refined FP64
      reciprocals; optionally (r >= r_f32) DP5(4) stages in FP32 with hardware reciprocals, the
      state accumulated in FP64, the error norm, step controller and radial-constraint projection
@@ -350,6 +353,7 @@ struct MixedParams {
     double r_far;     // far field (tolerance tol_far) at r >= r_far
     double r_f32;     // FP32 stages at r >= r_f32 (>= r_far; infinite: FP64 stages everywhere)
     double r_tol;     // scene tolerance below r_tol, tol_mid between r_tol and r_far
+    double r_cut;     // emission, absorption and the transfer step cap only below r_cut
     double tol_mid;   // tolerance between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
     EmisParams E;
@@ -404,7 +408,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     const bool flow = RADIO && P.flow_on;
     State n;
     double hc = R.h;
-    if (flow && R.s.y0 < P.r_flow_max) {
+    if (flow && R.s.y0 < M.r_cut) {
         double sn = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0));
         double cap = P.step_frac * fmin(R.s.y0 * rcpd(fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(R.s.y3) + 1e-30));
         hc = fmin(hc, cap);
@@ -443,7 +447,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         R.ppz = pnz;
     }
     if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
-    if (flow && n.y0 < P.r_flow_max) {
+    if (flow && n.y0 < M.r_cut) {
         ++R.flow_steps;
         FlowSample f;
         if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {

@@ -19,16 +19,17 @@ def frame_errors(a, b):
     return flux, float(np.median(pix)), float(np.percentile(pix, 99))
 
 
-@pytest.mark.parametrize("far_fp32", [0, 1])
-def test_mixed_radio_c2_error_bounds(far_fp32):
-    """Both stage-precision policies: FP64 stages everywhere (default) and FP32 stages outside the flow."""
+@pytest.mark.parametrize("far_fp32,full_flow", [(0, 0), (1, 0), (0, 1)])
+def test_mixed_radio_c2_error_bounds(far_fp32, full_flow):
+    """The policy variants: FP64 stages everywhere (default) or FP32 stages outside the flow; emission
+    inside the emission region (default) or over the whole flow."""
     cfg = setup(shot("c2_radio_1024.ini"))
     cfg["S"].n0 = 291464.1043528089   # the 0.6 Jy normalisation of this frame
     t = tracer_for(cfg)
     ref, st64 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_FP64))
-    mix, st32 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_MIXED, far_fp32=far_fp32))
+    mix, st32 = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_MIXED, far_fp32=far_fp32, full_flow=full_flow))
     flux, med, p99 = frame_errors(mix, ref)
-    print(f"mixed (far_fp32={far_fp32}) vs fp64: flux {flux:.3e} median {med:.3e} p99 {p99:.3e}; steps/ray "
+    print(f"mixed (far_fp32={far_fp32}, full_flow={full_flow}) vs fp64: flux {flux:.3e} median {med:.3e} p99 {p99:.3e}; steps/ray "
           f"{st32.step_attempts / 2**20:.1f} vs {st64.step_attempts / 2**20:.1f}, FP64 attempts "
           f"{st32.fp64_steps / st32.step_attempts:.2%}; kernel {st32.ms_kernel:.2f} vs {st64.ms_kernel:.2f} ms")
     assert flux < 1e-3
