@@ -32,8 +32,10 @@ int bhh_build_scene(const char* ini, const char* sets, bh_scene* scene, float* k
 void bhh_turbulence(int n, int nt, double half, double sigma, double corr_m, double corr_t, double dt,
                     uint32_t seed, float* out);
 
-/* Frames of a shot: 14 values per frame (t, r, incl, phi, roll, yaw, pitch, fov, fov_in_m, zoom,
-   target, off_x, off_y, time_m). Returns the frame count (< 0 on error). */
+/* Frames of a shot: BHH_SHOT_VALUES values per frame (t, r, incl, phi, roll, yaw, pitch, fov,
+   fov_in_m, zoom, target, off_x, off_y, time_m, blur, overlay, hide_direct, exposure_ev, fade, hud).
+   Returns the frame count (< 0 on error). */
+#define BHH_SHOT_VALUES 20
 int bhh_build_shot(const char* ini, const char* sets, double* out, int maxn);
 
 /* make_frame for explicit camera parameters (angles in radians). */
@@ -51,6 +53,15 @@ int bhh_create_from_ini(const char* ini, const char* sets, int device, int norma
 
 /* find_critical_point along psi for a static frame, on the context's spin. */
 int bhh_find_critical_point(bh_ctx* ctx, const bh_frame* frame, double psi, double* rho);
+
+/* 8-bit PNG of w x h pixels with `channels` (1, 3 or 4) bytes each, row-major from the top left,
+   Sub-filtered scanlines, zlib `level` (write_png, common/image_io.cpp:37-66). Returns 0 or -1. */
+int bhh_write_png(const char* path, int w, int h, int channels, const unsigned char* data, int level);
+
+/* Per-frame exposure from the frames' lum_percentile (main.cpp:304-319): out[i] = key /
+   exp(sum_j w_ij ln lp[j] / sum_j w_ij), w_ij = exp(-(i-j)^2 / 2 sigma^2) for |i-j| <= ceil(3 sigma)
+   and lp[j] > 0 (sigma = 0: the frame alone); out[i] = 1 where no frame contributes. */
+void bhh_smooth_exposure(const double* lp, int n, double key, double sigma, double* out);
 
 /* Self-test rows (8 x 6: a, incl_deg, traced -, traced +, analytic -, analytic +) on `device`. */
 int bhh_selftest(int device, double* rows, char* err, size_t errlen);

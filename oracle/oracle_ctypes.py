@@ -373,7 +373,7 @@ class Ref:
         return out, k2
 
     def build_shot(self, path, sets="", maxn=4096):
-        out = np.zeros((maxn, 14))
+        out = np.zeros((maxn, 20))
         n = self.lib.ref_build_shot(path.encode(), sets.encode(), _dp(out), maxn)
         return out[:max(n, 0)]
 

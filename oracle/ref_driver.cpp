@@ -115,9 +115,10 @@ int ref_build_shot(const char* path, const char* sets, double* out, int maxn) {
     int n = (int)sh.frames.size();
     for (int i = 0; i < n && i < maxn; ++i) {
         const ShotFrame& f = sh.frames[i];
-        double v[14] = {f.t, f.r, f.incl, f.phi, f.roll, f.yaw, f.pitch, f.fov, f.fov_in_m ? 1.0 : 0.0,
-                        f.zoom, f.target, f.off_x, f.off_y, f.time_m};
-        std::memcpy(out + 14 * i, v, sizeof v);
+        double v[20] = {f.t, f.r, f.incl, f.phi, f.roll, f.yaw, f.pitch, f.fov, f.fov_in_m ? 1.0 : 0.0,
+                        f.zoom, f.target, f.off_x, f.off_y, f.time_m, f.blur, f.overlay, f.hide_direct,
+                        f.exposure_ev, f.fade, f.hud};
+        std::memcpy(out + 20 * i, v, sizeof v);
     }
     return n;
 }

@@ -22,11 +22,12 @@ EXPORTS = [
     "bh_device_count", "bh_create", "bh_update_scene", "bh_get_scene", "bh_render", "bh_render_resident",
     "bh_trace_tile", "bh_radio_flux", "bh_vacuum_fates", "bh_math_eval", "bh_destroy", "bh_last_error",
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
-    "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest",
+    "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
     "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap",
 ]
 MLP_NPARAM = 8706
+SHOT_VALUES = 20   # values per frame of bhh_build_shot
 MLP_NNORM = 10   # input centres[3], half-widths[3], output means[2], scales[2]
 
 
@@ -210,6 +211,9 @@ def load():
                                       C.c_size_t]
     L.bhh_find_critical_point.restype = C.c_int
     L.bhh_find_critical_point.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_double, _D]
+    L.bhh_write_png.restype = C.c_int
+    L.bhh_write_png.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int]
+    L.bhh_smooth_exposure.argtypes = [_D, C.c_int, C.c_double, C.c_double, _D]
     L.bhh_selftest.restype = C.c_int
     L.bhh_selftest.argtypes = [C.c_int, _D, C.c_char_p, C.c_size_t]
     _lib = L
@@ -256,11 +260,27 @@ def turbulence(n=128, nt=1, half=60.0, sigma=0.3, corr_m=5.0, corr_t=30.0, dt=10
 
 
 def build_shot(ini, sets="", maxn=4096):
-    out = np.zeros((maxn, 14))
+    out = np.zeros((maxn, SHOT_VALUES))
     n = load().bhh_build_shot(ini.encode(), sets.encode(), out.ctypes.data_as(_D), maxn)
     if n < 0:
         raise BhError(f"build_shot failed for {ini}")
     return out[:n]
+
+
+def write_png(path, img, level=4):
+    """8-bit PNG of an H x W (grey), H x W x 3 or H x W x 4 uint8 array."""
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    if load().bhh_write_png(str(path).encode(), a.shape[1], a.shape[0], ch, a.ctypes.data_as(C.POINTER(C.c_uint8)),
+                            level) != 0:
+        raise BhError(f"cannot write {path}")
+
+
+def smooth_exposure(lp, key, sigma):
+    lp = np.ascontiguousarray(lp, dtype=np.float64)
+    out = np.zeros_like(lp)
+    load().bhh_smooth_exposure(lp.ctypes.data_as(_D), len(lp), key, sigma, out.ctypes.data_as(_D))
+    return out
 
 
 def make_frame(a, r, incl, phi=0.0, roll=0.0, yaw=0.0, pitch=0.0, fov=30.0, fov_in_m=True, zoom=1.0, target=0.0,

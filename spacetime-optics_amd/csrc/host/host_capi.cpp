@@ -5,6 +5,7 @@
 
 #include "../../../include/bhrt_host.h"
 #include "config.h"
+#include "image_io.h"
 #include "renderer.h"
 #include "scene.h"
 #include "shot.h"
@@ -97,9 +98,10 @@ int bhh_build_shot(const char* ini, const char* sets, double* out, int maxn) {
     int n = (int)sh.frames.size();
     for (int i = 0; i < n && i < maxn; ++i) {
         const bh::ShotFrame& f = sh.frames[i];
-        double v[14] = {f.t, f.r, f.incl, f.phi, f.roll, f.yaw, f.pitch, f.fov, f.fov_in_m ? 1.0 : 0.0,
-                        f.zoom, f.target, f.off_x, f.off_y, f.time_m};
-        std::memcpy(out + 14 * (size_t)i, v, sizeof v);
+        double v[BHH_SHOT_VALUES] = {f.t, f.r, f.incl, f.phi, f.roll, f.yaw, f.pitch, f.fov, f.fov_in_m ? 1.0 : 0.0,
+                                     f.zoom, f.target, f.off_x, f.off_y, f.time_m, f.blur, f.overlay, f.hide_direct,
+                                     f.exposure_ev, f.fade, f.hud};
+        std::memcpy(out + BHH_SHOT_VALUES * (size_t)i, v, sizeof v);
     }
     return n;
 }
@@ -205,6 +207,17 @@ int bhh_create_from_ini(const char* ini, const char* sets, int device, int norma
     }
     *ctx = R.release();
     return BH_OK;
+}
+
+int bhh_write_png(const char* path, int w, int h, int channels, const unsigned char* data, int level) {
+    if (!path) return -1;
+    return bh::write_png(path, w, h, channels, data, level) ? 0 : -1;
+}
+
+void bhh_smooth_exposure(const double* lp, int n, double key, double sigma, double* out) {
+    if (!lp || !out || n <= 0) return;
+    const std::vector<double> e = bh::smooth_exposure(std::vector<double>(lp, lp + n), key, sigma);
+    for (int i = 0; i < n; ++i) out[i] = e[(size_t)i];
 }
 
 int bhh_selftest(int device, double* rows, char* err, size_t errlen) {

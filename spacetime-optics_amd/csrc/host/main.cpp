@@ -1,11 +1,14 @@
 /* bhrt - command line front end on the MI355X tracer (blackholes/src/main.cpp:374-626 of the
    reference, single GPU): bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B]
    [--list] [--selftest]. Frames are rendered as the reference's Renderer::render does (adaptive
-   sampling between render.spp and render.spp_max) and written as linear HDR (I, I_n1, I_n2,
-   I_n>=3) float images "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first; tone mapping, HUD
-   and PNG output are not part of this path. A shot with a [batch] section (frames, spins, sizes,
-   frame_dt_m) is traced in one persistent launch instead: frame i has spin spins[i % #spins],
-   size sizes[(i / #spins) % #sizes] squared and time i * frame_dt_m. */
+   sampling between render.spp and render.spp_max, then in radio mode the beam blur of post.blur x
+   post.beam_uas and the frame's display statistics); the exposure of every frame follows from the
+   smoothed statistics of all frames (post.exposure*, main.cpp:569-576), and each frame is tone
+   mapped and written as out/frames/frame_NNNNN.png (main.cpp:578-603). The HUD text overlay is not
+   drawn. With output.hdr = 1 the linear HDR frames (I, I_n1, I_n2, I_n>=3) are written as well:
+   "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first. A shot with a [batch] section (frames,
+   spins, sizes, frame_dt_m) is traced in one persistent launch instead: frame i has spin
+   spins[i % #spins], size sizes[(i / #spins) % #sizes] squared and time i * frame_dt_m. */
 #include <sys/stat.h>
 
 #include <chrono>
@@ -18,6 +21,7 @@
 
 #include "../../../include/bhrt_host.h"
 #include "config.h"
+#include "image_io.h"
 #include "renderer.h"
 #include "scene.h"
 #include "shot.h"
@@ -46,6 +50,66 @@ bool write_frame(const std::string& path, const bh::HdrFrame& f) {
     bool ok = std::fwrite("BHF1", 1, 4, fp) == 4 && std::fwrite(hdr, 4, 3, fp) == 3 &&
               std::fwrite(f.px.data(), 4, f.px.size(), fp) == f.px.size();
     return std::fclose(fp) == 0 && ok;
+}
+
+// Phase 2 of the reference's frame loop (main.cpp:569-603): exposure from the smoothed display
+// statistics of all frames, radio tone map, PNG per frame (and the HDR frame with output.hdr = 1).
+// Visible-mode frames (vacuum geodesics: the disk and sky shading are not part of this tracer) are
+// written as HDR only.
+bool tone_and_write(const bh::Config& cfg, const bh::Shot& shot, bh::Renderer& R, bool radio,
+                    const std::vector<int>& index, const std::vector<bh::HdrFrame>& frames, const std::string& out) {
+    const size_t n = frames.size();
+    if (!radio) {
+        for (size_t i = 0; i < n; ++i) {
+            char name[64];
+            std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", index[i]);
+            if (!write_frame(out + name, frames[i])) {
+                std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+                return false;
+            }
+        }
+        return true;
+    }
+    std::vector<double> lp(n);
+    for (size_t i = 0; i < n; ++i) lp[i] = frames[i].lum_percentile;
+    const double key = cfg.num("post.exposure_key", 0.92);
+    const double sigma = std::min(cfg.num("post.exposure_smooth", 4.0), (double)shot.frames.size() / 30.0);
+    std::vector<double> ex = bh::smooth_exposure(lp, key, sigma);
+    if (cfg.str("post.exposure", "auto") != "auto") ex.assign(n, key / cfg.num("post.exposure", 1.0));
+    const bool hdr = cfg.flag("output.hdr", false);
+    std::vector<uint8_t> rgb;
+    for (size_t i = 0; i < n; ++i) {
+        const int idx = index[i];
+        const bh::ShotFrame& f = shot.frames[(size_t)std::min<size_t>((size_t)idx, shot.frames.size() - 1)];
+        bh_tone_params tp{};
+        tp.exposure = (float)(ex[i] * std::pow(2.0, f.exposure_ev));
+        tp.colormap = cfg.str("post.colormap", "afmhot") == "inferno" ? 1 : 0;
+        tp.gamma = (float)cfg.num("post.gamma", 1.0);
+        tp.overlay = (float)f.overlay;
+        tp.hide_direct = (float)f.hide_direct;
+        tp.fade = (float)f.fade;
+        tp.vignette = (float)cfg.num("post.vignette", 0.15);
+        tp.grain = (float)cfg.num("post.grain", 0.006);
+        tp.seed = 0x51ED27u + (uint32_t)idx * 7919u;
+        if (!R.tonemap(frames[i], tp, &rgb)) {
+            std::fprintf(stderr, "bhrt: tone map of frame %d: %s\n", idx, R.error().c_str());
+            return false;
+        }
+        char name[64];
+        std::snprintf(name, sizeof name, "/frames/frame_%05d.png", idx);
+        if (!bh::write_png(out + name, frames[i].W, frames[i].H, 3, rgb.data())) {
+            std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+            return false;
+        }
+        if (hdr) {
+            std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", idx);
+            if (!write_frame(out + name, frames[i])) {
+                std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+                return false;
+            }
+        }
+    }
+    return true;
 }
 
 int selftest() {
@@ -219,42 +283,55 @@ int main(int argc, char** argv) {
             return 1;
         }
         const double tb = wall() - t1;
+        std::vector<int> index((size_t)nb);
+        const double percentile = cfg.num("post.exposure_percentile", 0.995);
         for (int i = 0; i < nb; ++i) {
-            char name[64];
-            std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", i);
-            if (!write_frame(out + name, imgs[(size_t)i])) {
-                std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+            index[(size_t)i] = i;
+            bh::HdrFrame& h = imgs[(size_t)i];
+            bh_post_params pp{0.0, 0.f};
+            if (bh_post_radio(R.ctx(), h.px.data(), h.W, h.H, &pp, percentile, &h.lum_percentile, &h.lum_max) != BH_OK) {
+                std::fprintf(stderr, "bhrt: batch post: %s\n", R.error().c_str());
                 return 1;
             }
         }
+        if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, imgs, out)) return 1;
         std::printf("done: batch of %d frames, %.3g rays in %.1f ms kernel / %.2f s wall (%.1f Mrays/s)\n", nb,
                     (double)st.samples, st.ms_kernel, tb, st.samples / std::max(st.ms_kernel, 1e-9) * 1e-3);
         return 0;
     }
     long long samples = 0, limited = 0;
     double t_render = wall();
+    // phase 1: render with the post step (main.cpp:519-564)
+    const double percentile = cfg.num("post.exposure_percentile", 0.995);
+    const double beam_px_uas = cfg.num("post.beam_uas", 20.0);
+    std::vector<bh::HdrFrame> done;
+    std::vector<int> index;
     for (int idx = fa; idx < fb; ++idx) {
-        bh_frame F = bh::make_frame(shot, shot.frames[idx], S.a, rs.W, rs.H, rs.spp, 0x9E3779B9u * (uint32_t)(idx + 1));
+        const bh::ShotFrame& sf = shot.frames[idx];
+        bh_frame F = bh::make_frame(shot, sf, S.a, rs.W, rs.H, rs.spp, 0x9E3779B9u * (uint32_t)(idx + 1));
+        bh_post_params pp{0.0, (float)sf.hide_direct};
+        if (S.mode == BH_MODE_RADIO && sf.blur > 0 && sf.fov_in_m) {
+            const double pix_uas = sf.fov / sf.zoom * si.uas_per_m / rs.W;
+            pp.blur_sigma_px = sf.blur * beam_px_uas / 2.3548 / pix_uas;
+        }
         bh::HdrFrame hdr;
         bh_stats st{};
         double t1 = wall();
-        if (!R.render(F, o, &hdr, &st)) {
+        if (!R.render(F, o, pp, percentile, &hdr, &st)) {
             std::fprintf(stderr, "bhrt: frame %d: %s\n", idx, R.error().c_str());
             return 1;
         }
         samples += st.samples;
         limited += st.step_limited;
-        char name[64];
-        std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", idx);
-        if (!write_frame(out + name, hdr)) {
-            std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
-            return 1;
-        }
         std::printf("[gpu 0] %s frame %4d  %6.1f Msamples  %7.1f ms  (step-limited %lld)\n", shot.name.c_str(), idx,
                     st.samples * 1e-6, (wall() - t1) * 1e3, st.step_limited);
         std::fflush(stdout);
+        done.push_back(std::move(hdr));
+        index.push_back(idx);
     }
     double tr = wall() - t_render;
+    // phase 2: exposure, tone map, PNG
+    if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, done, out)) return 1;
     FILE* m = std::fopen((out + "/meta.json").c_str(), "w");
     if (m) {
         std::fprintf(m, "{\"shot\": \"%s\", \"frames\": %d, \"fps\": %g, \"width\": %d, \"height\": %d}\n", shot.name.c_str(),

@@ -40,6 +40,39 @@ bool Renderer::render(const bh_frame& f, const bh_trace_opts& opts, HdrFrame* ou
     return bh_render(ctx_, &f, &opts, out->px.data(), st) == BH_OK;
 }
 
+bool Renderer::render(const bh_frame& f, const bh_trace_opts& opts, const bh_post_params& post, double percentile,
+                      HdrFrame* out, bh_stats* st) {
+    if (!render(f, opts, out, st)) return false;
+    bh_post_params pp = post;
+    if (S_.mode != BH_MODE_RADIO) pp.blur_sigma_px = 0.0;   // the beam blur is a radio-mode filter
+    return bh_post_radio(ctx_, out->px.data(), out->W, out->H, &pp, percentile, &out->lum_percentile,
+                         &out->lum_max) == BH_OK;
+}
+
+bool Renderer::tonemap(const HdrFrame& in, const bh_tone_params& tp, std::vector<uint8_t>* rgb8) {
+    if (!ctx_) return false;
+    rgb8->resize((size_t)in.W * in.H * 3);
+    return bh_tonemap(ctx_, in.px.data(), in.W, in.H, &tp, rgb8->data()) == BH_OK;
+}
+
+std::vector<double> smooth_exposure(const std::vector<double>& lp, double key, double sigma) {
+    const long n = (long)lp.size();
+    std::vector<double> out((size_t)n, 1.0);
+    const int r = (int)std::ceil(3 * sigma);
+    for (long i = 0; i < n; ++i) {
+        double ws = 0, acc = 0;
+        for (int d = -r; d <= r; ++d) {
+            const long j = i + d;
+            if (j < 0 || j >= n || lp[(size_t)j] <= 0) continue;
+            const double w = sigma > 0 ? std::exp(-0.5 * d * d / (sigma * sigma)) : (d == 0 ? 1.0 : 0.0);
+            ws += w;
+            acc += w * std::log(lp[(size_t)j]);
+        }
+        if (ws > 0) out[(size_t)i] = key / std::exp(acc / ws);
+    }
+    return out;
+}
+
 bool Renderer::render_batch(const std::vector<bh_batch_frame>& frames, const bh_trace_opts& opts,
                             std::vector<HdrFrame>* out, bh_stats* st) {
     if (!ctx_ || frames.empty()) return false;

@@ -15,6 +15,7 @@ namespace bh {
 struct HdrFrame {
     int W = 0, H = 0;
     std::vector<float> px;
+    double lum_percentile = 0, lum_max = 0;   // display statistics of the post step
 };
 
 class Renderer {
@@ -30,6 +31,12 @@ public:
     // more than spp samples (spp_max > spp), else the plain mean of spp samples. opts selects
     // precision / emissivity / jitter.
     bool render(const bh_frame& f, const bh_trace_opts& opts, HdrFrame* out, bh_stats* st);
+    // Renderer::render with its post step (render.cu:755-792): the trace above, then in radio mode
+    // the beam blur (post.blur_sigma_px > 0.5) and the display statistics at `percentile`.
+    bool render(const bh_frame& f, const bh_trace_opts& opts, const bh_post_params& post, double percentile,
+                HdrFrame* out, bh_stats* st);
+    // Renderer::tonemap (render.cu:811-846) of a radio frame to 8-bit RGB, without the HUD overlay.
+    bool tonemap(const HdrFrame& in, const bh_tone_params& tp, std::vector<uint8_t>* rgb8);
     // One persistent launch over several frames (each with its own scene): pixel-centre or sample-0
     // rays, images one after the other in `out`.
     bool render_batch(const std::vector<bh_batch_frame>& frames, const bh_trace_opts& opts,
@@ -52,6 +59,10 @@ private:
     bh_scene S_{};
     bh_settings rs_{};
 };
+
+/* Per-frame exposure (main.cpp:304-319): key / exp(Gaussian-weighted mean of ln lum_percentile over
+   frames within 3 sigma, frames with lp <= 0 skipped); 1 where no frame contributes. */
+std::vector<double> smooth_exposure(const std::vector<double>& lp, double key, double sigma);
 
 /* Flux normalisation (main.cpp:475-508): secant iteration on ln n0 until the quick render of
    frame f0 (zoom 1, no target/offset, fov = info.normalize_fov_m, 160^2, spp 2) gives

@@ -120,6 +120,18 @@ bool build_shot(const Config& c, Shot* shot, std::string* err) {
         f.off_x = param(c, "camera.offset_x_m", 0, t, false);
         f.off_y = param(c, "camera.offset_y_m", 0, t, false);
         f.time_m = param(c, "time.t_m", 0, t, false);
+        // post-processing timeline (shot.cpp:89-99): beam blur, order tint, direct-image removal,
+        // exposure offset, fades, HUD opacity
+        f.blur = param(c, "post.blur", 0, t, false);
+        f.overlay = param(c, "post.overlay", 0, t, false);
+        f.hide_direct = param(c, "post.hide_direct", 0, t, false);
+        f.exposure_ev = param(c, "post.exposure_ev", 0, t, false);
+        const double fade_in = c.num("shot.fade_in", 0.0), fade_out = c.num("shot.fade_out", 0.0);
+        double fade = param(c, "post.fade", 1, t, false);
+        if (fade_in > 0) fade *= std::min(1.0, t / fade_in);
+        if (fade_out > 0) fade *= std::min(1.0, (shot->duration - t) / fade_out);
+        f.fade = std::max(0.0, fade);
+        f.hud = param(c, "hud.alpha", 1, t, false);
         shot->frames.push_back(f);
     }
     return true;

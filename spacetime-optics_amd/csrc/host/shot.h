@@ -25,6 +25,7 @@ struct ShotFrame {
     double fov = 0;                    // M (fov_in_m) or radians
     bool fov_in_m = false;
     double zoom = 1, target = 0, off_x = 0, off_y = 0, time_m = 0;
+    double blur = 0, overlay = 0, hide_direct = 0, exposure_ev = 0, fade = 1, hud = 1;   // post timeline
 };
 
 struct Shot {

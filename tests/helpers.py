@@ -3,6 +3,8 @@ and the oracle, built once by the product's host layer and copied field by field
 import ctypes as C
 import math
 import os
+import struct
+import zlib
 
 import numpy as np
 
@@ -67,3 +69,28 @@ def tracer_for(cfg, device=0):
 def pixel_grid(w, h, x0=0, y0=0):
     py, px = np.mgrid[y0:y0 + h, x0:x0 + w]
     return px.ravel().astype(np.int32), py.ravel().astype(np.int32)
+
+
+def read_png(path):
+    """Decodes an 8-bit PNG written with Sub-filtered rows, checking every chunk's CRC."""
+    data = open(path, "rb").read()
+    assert data[:8] == bytes([137, 80, 78, 71, 13, 10, 26, 10])
+    pos, chunks = 8, []
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(typ + body) & 0xFFFFFFFF, typ
+        chunks.append((typ, body))
+        pos += 12 + n
+    assert chunks[0][0] == b"IHDR" and chunks[-1][0] == b"IEND"
+    w, h, depth, ctype, comp, filt, inter = struct.unpack(">IIBBBBB", chunks[0][1])
+    assert (depth, comp, filt, inter) == (8, 0, 0, 0)
+    ch = {0: 1, 2: 3, 6: 4}[ctype]
+    raw = zlib.decompress(b"".join(b for t, b in chunks if t == b"IDAT"))
+    rows = np.frombuffer(raw, dtype=np.uint8).reshape(h, 1 + w * ch)
+    out = np.zeros((h, w * ch), dtype=np.uint8)
+    for y in range(h):
+        assert rows[y, 0] == 1   # Sub filter: each byte adds the reconstructed byte one pixel left
+        out[y] = (np.cumsum(rows[y, 1:].astype(np.int64).reshape(w, ch), axis=0) & 255).ravel()
+    return out.reshape(h, w, ch) if ch > 1 else out.reshape(h, w)
