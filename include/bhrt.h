@@ -194,9 +194,9 @@ typedef struct bh_batch_frame {
 
 typedef struct bh_batch_cfg {
     int block;          /* threads per workgroup: 64, 128 or 256 (0 -> 256) */
-    int grid_per_cu;    /* resident workgroups per compute unit (0 -> 4) */
-    int interval;       /* integration steps between retire/refill (0 -> 32) */
-    int refill_min;     /* free lanes needed to refill (0 -> 32) */
+    int grid_per_cu;    /* resident workgroups per compute unit (0 -> the default) */
+    int interval;       /* integration steps between retire/refill (0 -> the default) */
+    int refill_min;     /* free lanes needed to refill (0 -> the default) */
     int use_graph;      /* capture and replay the launch as a hipGraph */
     int waves;          /* 0: kernel compiled for the compiler's register budget; 3 or 4: compiled for at
                            least that many resident waves per SIMD (fewer registers) */
@@ -205,8 +205,8 @@ typedef struct bh_batch_cfg {
                            (the rays that orbit longest start first); 0 -> the default */
     int prio_age;       /* > 0: a wave holding a ray that has taken prio_age steps raises its issue
                            priority; < 0: off; 0 -> the default. A NULL cfg (and no autotuned one)
-                           means 256 / 4 / 32 / 32 / graph / 3 with, for the mixed tracer, impact
-                           order and prio_age 256, for the FP64 tracer row order and no priority. */
+                           means, for the mixed tracer, 256 / 4 / 32 / 32 / graph / 0 / impact order /
+                           256, for the FP64 tracer 256 / 8 / 64 / 48 / graph / 3 / row order / 256. */
 } bh_batch_cfg;
 
 /* Traces every frame of `frames`; px_out (nullable) receives the frames one after the other,

@@ -742,11 +742,16 @@ PersistFn pick_persist_w(int prec, bool radio) {
     return radio ? k_persist<BH_PREC_MIXED, true, W, BW> : k_persist<BH_PREC_MIXED, false, W, BW>;
 }
 
-// adaptive passes: kernels compiled for 3 waves per SIMD (the autotuned choice on both precisions)
+// adaptive passes: the kernel compiled for at least 3 waves per SIMD (waves 3 or 4) or for the
+// compiler's register budget (waves 0)
+template <int W, class Work>
+void (*pick_adaptive_w(int prec, bool radio))(const Work, unsigned int*, int, int, int, BlockOut*) {
+    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, W, Work> : k_persist<BH_PREC_FP64, false, W, Work>;
+    return radio ? k_persist<BH_PREC_MIXED, true, W, Work> : k_persist<BH_PREC_MIXED, false, W, Work>;
+}
 template <class Work>
-void (*pick_adaptive(int prec, bool radio))(const Work, unsigned int*, int, int, int, BlockOut*) {
-    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, 3, Work> : k_persist<BH_PREC_FP64, false, 3, Work>;
-    return radio ? k_persist<BH_PREC_MIXED, true, 3, Work> : k_persist<BH_PREC_MIXED, false, 3, Work>;
+void (*pick_adaptive(int prec, bool radio, int waves))(const Work, unsigned int*, int, int, int, BlockOut*) {
+    return waves >= 3 ? pick_adaptive_w<3, Work>(prec, radio) : pick_adaptive_w<1, Work>(prec, radio);
 }
 
 PersistFn pick_persist(int prec, bool radio, int waves) {
@@ -759,20 +764,23 @@ PersistFn pick_persist(int prec, bool radio, int waves) {
 
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
 
-// Defaults, measured on the 1024^2 230 GHz frame (MI355X). The mixed tracer's launch is bound by its
-// longest rays (up to 7x the mean step count): queueing them first (impact order) and raising the
-// priority of waves that hold rays older than 256 steps cut its frame time by a quarter. The FP64
-// tracer (at most 3x the mean) is throughput-bound and keeps row order without priorities.
+// Defaults: the autotuner's choices on the 1024^2 230 GHz frame (MI355X). The mixed tracer's launch is
+// bound by its longest rays (up to 7x the mean step count): queueing them first (impact order) and
+// raising the priority of waves that hold rays older than 256 steps cut its frame time by a quarter,
+// and with that the compiler's own register budget (2 waves per SIMD, no spills) beats a forced 3.
+// The FP64 tracer (at most 3x the mean) is throughput-bound: 3 waves per SIMD, more resident
+// workgroups and longer retire intervals, row order.
 bh_batch_cfg batch_defaults(int prec) {
     bh_batch_cfg d{};
+    const bool mixed = prec == BH_PREC_MIXED;
     d.block = 256;
-    d.grid_per_cu = 4;
-    d.interval = 32;
-    d.refill_min = 32;
+    d.grid_per_cu = mixed ? 4 : 8;
+    d.interval = mixed ? 32 : 64;
+    d.refill_min = mixed ? 32 : 48;
     d.use_graph = 1;
-    d.waves = 3;
-    d.order = prec == BH_PREC_MIXED ? BH_ORDER_IMPACT : BH_ORDER_ROWS;
-    d.prio_age = prec == BH_PREC_MIXED ? 256 : -1;
+    d.waves = mixed ? 0 : 3;
+    d.order = mixed ? BH_ORDER_IMPACT : BH_ORDER_ROWS;
+    d.prio_age = 256;
     return d;
 }
 
@@ -1011,7 +1019,7 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     // pass 1
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
     if (ordered) enqueue_tile_order(c->st, dB + 1, 1, (unsigned int)tiles, c->d_ohist);
-    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
+    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, radio, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
                        bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, prio,
                        c->d_blk);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
@@ -1047,7 +1055,7 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
             HIPCK(c, hipMemcpyAsync(c->d_bframes + sizeof(bhp::BatchFrame), &B[1], sizeof B[1], hipMemcpyHostToDevice,
                                     c->st));
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
-            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, radio), dim3(grid), dim3(cfg.block), 0, c->st,
+            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, radio, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
                                bhp::ChunkWork{dB + 1, 1, spp, npix, c->d_offsets, c->d_samp2}, c->d_queue,
                                cfg.interval, cfg.refill_min, prio, c->d_blk);
             hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
