@@ -44,16 +44,25 @@ typedef struct bh_scene {
     int mode;               /* BH_MODE_* ; visible = vacuum geodesics only */
     int flow_on;
     double n0, n_index, theta_index, theta0, h_r, beta_p, r_flow_max, nu_obs, rg_cm, flow_kep;
-    float flow_turb_amp;    /* > 0 enables the turbulence grid of bh_tables */
+    float flow_turb_amp;    /* > 0 enables the turbulence (bh_tables.turb_model) */
     uint32_t seed;
+    float flow_turb_scale;  /* sheared turbulence: feature size in units of R (flow.turbulence_scale) */
+    float flow_tcorr;       /* sheared turbulence: correlation time in orbital periods (flow.correlation_orbits) */
 } bh_scene;
 
 /* SceneTables (renderer.hpp:19-23) plus the turbulence input. The K2 table is ln K2(1/Theta) on a
-   ln Theta grid (scene_build.cpp:159-170). The turbulence grid replaces the baked texture slices
-   (render.cu:448-460, emission.cuh:112-123): turb_nt time slices of a cell-centred turb_n^3
-   Cartesian grid over [-turb_half, turb_half]^3 (x fastest), holding the density factor. With
-   turb_nt > 1, slice k sits at coordinate time turb_t0 + k*turb_dt and the tracer interpolates
-   linearly in the emission time of each step (slow light). */
+   ln Theta grid (scene_build.cpp:159-170). Two turbulence models (density factor of the flow):
+   - BH_TURB_SHEARED, the reference's: sheared, finite-lifetime fBm noise (emission.cuh:30-51)
+     baked by the library on the device for each frame's camera time into ts_dims[3] slices of
+     retarded time, each ts_dims[2] x ts_dims[1] x ts_dims[0] cells over (z/R, ln R, phi)
+     (render.cu:447-460,594-619; 0 -> the reference's 512 x 192 x 48 x 24), read with trilinear
+     filtering (wrap in phi, clamp elsewhere, weights rounded to 1/256 like a texture unit) and
+     interpolated between slices (emission.cuh:112-123). turb is ignored.
+   - BH_TURB_GRID: turb_nt time slices of a cell-centred turb_n^3 Cartesian grid over
+     [-turb_half, turb_half]^3 (x fastest) given by the caller. With turb_nt > 1, slice k sits at
+     coordinate time turb_t0 + k*turb_dt and the tracer interpolates linearly in the emission time
+     of each step (slow light). */
+enum { BH_TURB_GRID = 0, BH_TURB_SHEARED = 1 };
 typedef struct bh_tables {
     const float* k2_ln;
     int k2_n;
@@ -61,6 +70,8 @@ typedef struct bh_tables {
     const float* turb;
     int turb_n, turb_nt;
     double turb_half, turb_t0, turb_dt;
+    int turb_model;         /* BH_TURB_* */
+    int ts_dims[4];         /* sheared: phi, ln R, z/R cells and time slices (0 -> 512, 192, 48, 24) */
 } bh_tables;
 
 /* FrameGPU (scene.hpp:71-80). kp = KerrPoint at the camera (kerr.hpp:20-25): r, sin th, cos th,
@@ -269,6 +280,12 @@ int bh_mlp_grad(bh_ctx* ctx, const float* params, const float* norm, const float
    points (|d ln f| > judge_tol, 0 -> 2e-3, fails it). The mixed tracer with BH_EMIS_MLP reads this
    table and uses the exact formula in failed cells and outside the box. */
 int bh_mlp_bake(bh_ctx* ctx, int n0, int n1, int n2, float judge_tol, bh_mlp_bake_report* report);
+
+/* The sheared turbulence slices for a frame's camera (bh_tables.turb_model = BH_TURB_SHEARED): bakes
+   them as a trace of that frame would (render.cu:679-682) and copies them to out, which holds
+   ts_dims[3] x ts_dims[2] x ts_dims[1] x ts_dims[0] floats (slice, z/R, ln R, phi; phi fastest);
+   n = the number of floats out can hold. */
+int bh_sheared_slices(bh_ctx* ctx, const bh_frame* frame, float* out, size_t n);
 
 /* ---- Radio post-processing and display (the radio half of Renderer::render's post step,
    render.cu:742-753,756-792, and of Renderer::tonemap, render.cu:811-846, without the HUD overlay).

@@ -5,8 +5,8 @@
    -ffp-contract=off it rounds identically), citing the reference file:line it restates. Paths are
    relative to the reference repository root. Out of scope and omitted: the thin disk
    (trace.cuh:106-128, emission.cuh:53-86), the sky texture (trace.cuh:35-43,154-160) and the
-   inline fBm turbulence (emission.cuh:30-51,124-127); the baked turbulence textures
-   (emission.cuh:112-123) are replaced by a software-trilinear Cartesian density grid.
+   inline fBm turbulence (emission.cuh:124-127). Turbulence: the reference's baked sheared noise
+   (sheared.cpp) or a software-trilinear Cartesian density grid.
 
    ORACLE_BHM selects the elementary functions: 0 -> glibc (reference host build), 1 -> bh_math.h
    (the functions the GPU tracer uses). sqrt, division and fmin/fmax are IEEE on both sides. */
@@ -442,7 +442,10 @@ void or_flow_step(const or_scene* S, double r, double u, double phi, double t, d
     double vert = M_EXP(-0.5 * zh * zh);
     if (vert < 1e-7) return;
     double ne = S->n0 * M_EXP(-S->n_index * M_LOG(r)) * vert;
-    if (S->flow_turb_amp > 0.f && S->turb_grid && S->turb_n > 1) ne *= or_turb_factor(S, r, u, phi, t);
+    if (S->flow_turb_amp > 0.f && S->turb_model == 1 && S->ts_slices)
+        ne *= (double)or_sheared_factor(S, r, R, z, phi, t);
+    else if (S->flow_turb_amp > 0.f && S->turb_grid && S->turb_n > 1)
+        ne *= or_turb_factor(S, r, u, phi, t);
     ne *= 1.0 - M_EXP(-(r - S->rh) * (r - S->rh) / 0.25);
     if (ne < 1e-30) return;
     double Th = S->theta0 * M_EXP(-S->theta_index * M_LOG(r));

@@ -18,6 +18,14 @@
 extern "C" {
 #endif
 
+// Parameters of one bake of the sheared turbulence (sheared.cpp; render.cu:615-619)
+typedef struct {
+    double a, r_isco, cam_t, cam_r;
+    float lr0, lr1, zmax, d0, dd, amp, scale, tcorr;
+    uint32_t seed;   // scene seed ^ 0x51f15eed
+    int nphi, nlr, nz, nt;
+} or_sheared;
+
 typedef struct {
     double a, rh, r_isco;
     double tol;
@@ -39,6 +47,11 @@ typedef struct {
     int k2_n;
     float k2_l0, k2_dl;
     uint32_t seed;
+    float flow_turb_scale, flow_tcorr;
+    // Sheared turbulence (turb_model 1): slices baked by or_sheared_bake with parameters ts
+    int turb_model;
+    const float* ts_slices;
+    or_sheared ts;
 } or_scene;
 
 typedef struct {
@@ -133,6 +146,18 @@ void or_math_eval(int fn, const double* x, const double* y, double* out, int n);
 // Self-test restatement (main.cpp:323-372): fills 8 rows of (a, incl, traced -, traced +,
 // analytic -, analytic +) and the unprojected radial-constraint drift.
 void or_selftest(double rows[8][6], double* drift, int* drift_steps);
+
+/* Sheared turbulence (sheared.cpp; emission.cuh:20-51,113-122, device_util.cuh:15-71,
+   render.cu:449-460,615-619). dims: phi, ln R, z/R cells and slices (NULL or 0 -> 512, 192, 48, 24).
+   or_sheared_bake fills nt slices of nz x nlr x nphi floats (phi fastest). */
+void or_sheared_params(const or_scene* S, double cam_t, double cam_r, const int dims[4], or_sheared* B);
+float or_sheared_gnoise(float x, float y, float z, uint32_t seed);
+float or_sheared_fbm(float x, float y, float z, int octaves, uint32_t seed);
+float or_sheared_noise(const or_sheared* B, double R, double phi, double z, double t);
+float or_sheared_tex(const float* T, int nx, int ny, int nz, float x, float y, float z);
+float or_sheared_cell(const or_sheared* B, int i, int j, int l, int k);
+void or_sheared_bake(const or_sheared* B, float* out, int nthreads);
+float or_sheared_factor(const or_scene* S, double r, double R, double z, double phi, double t);
 
 /* Radio post step and tone map (post.cpp; render.cu:244-407,742-791): beam blur in place on px
    (W*H*4 floats), the displayed-intensity statistics, the radio tone map to W*H*3 bytes. */

@@ -11,6 +11,15 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+class OrSheared(C.Structure):
+    _fields_ = [
+        ("a", C.c_double), ("r_isco", C.c_double), ("cam_t", C.c_double), ("cam_r", C.c_double),
+        ("lr0", C.c_float), ("lr1", C.c_float), ("zmax", C.c_float), ("d0", C.c_float), ("dd", C.c_float),
+        ("amp", C.c_float), ("scale", C.c_float), ("tcorr", C.c_float), ("seed", C.c_uint32),
+        ("nphi", C.c_int), ("nlr", C.c_int), ("nz", C.c_int), ("nt", C.c_int),
+    ]
+
+
 class OrScene(C.Structure):
     _fields_ = [
         ("a", C.c_double), ("rh", C.c_double), ("r_isco", C.c_double), ("tol", C.c_double),
@@ -22,7 +31,8 @@ class OrScene(C.Structure):
         ("turb_grid", C.POINTER(C.c_float)), ("turb_n", C.c_int), ("turb_half", C.c_double),
         ("turb_inv_dx", C.c_double), ("turb_nt", C.c_int), ("turb_t0", C.c_double), ("turb_inv_dt", C.c_double),
         ("k2_lntab", C.POINTER(C.c_float)), ("k2_n", C.c_int), ("k2_l0", C.c_float), ("k2_dl", C.c_float),
-        ("seed", C.c_uint32),
+        ("seed", C.c_uint32), ("flow_turb_scale", C.c_float), ("flow_tcorr", C.c_float),
+        ("turb_model", C.c_int), ("ts_slices", C.POINTER(C.c_float)), ("ts", OrSheared),
     ]
 
 
@@ -107,6 +117,21 @@ class Oracle:
         L.or_turb_factor.restype = C.c_double
         L.or_turb_factor.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_flow_step.argtypes = [C.POINTER(OrScene)] + [C.c_double] * 7 + [_D, _D]
+        _SH = C.POINTER(OrSheared)
+        L.or_sheared_params.argtypes = [C.POINTER(OrScene), C.c_double, C.c_double, C.POINTER(C.c_int), _SH]
+        L.or_sheared_gnoise.restype = C.c_float
+        L.or_sheared_gnoise.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint32]
+        L.or_sheared_fbm.restype = C.c_float
+        L.or_sheared_fbm.argtypes = [C.c_float, C.c_float, C.c_float, C.c_int, C.c_uint32]
+        L.or_sheared_noise.restype = C.c_float
+        L.or_sheared_noise.argtypes = [_SH, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.or_sheared_tex.restype = C.c_float
+        L.or_sheared_tex.argtypes = [_F, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.or_sheared_cell.restype = C.c_float
+        L.or_sheared_cell.argtypes = [_SH, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_sheared_bake.argtypes = [_SH, _F, C.c_int]
+        L.or_sheared_factor.restype = C.c_float
+        L.or_sheared_factor.argtypes = [C.POINTER(OrScene)] + [C.c_double] * 5
         L.or_post_blur.argtypes = [_F, C.c_int, C.c_int, C.c_double]
         L.or_frame_stats.argtypes = [_F, C.c_int, C.c_int, C.c_float, C.c_double, _D, _D]
         L.or_tonemap_radio.argtypes = ([_F, C.c_int, C.c_int, C.c_int] + [C.c_float] * 7 +
@@ -230,6 +255,47 @@ class Oracle:
                                             nthreads, img.ctypes.data_as(_D),
                                             ns.ctypes.data_as(C.POINTER(C.c_uint32)))
         return img, ns, total
+
+    # --- sheared turbulence (sheared.cpp) ------------------------------------------------------
+    def sheared_params(self, S, cam_t, cam_r, dims=(0, 0, 0, 0)):
+        B = OrSheared()
+        d = (C.c_int * 4)(*[int(v) for v in dims])
+        self.lib.or_sheared_params(C.byref(S), cam_t, cam_r, d, C.byref(B))
+        return B
+
+    def sheared_bake(self, B, nthreads=0):
+        """nt x nz x nlr x nphi float32 slices."""
+        out = np.zeros((B.nt, B.nz, B.nlr, B.nphi), dtype=np.float32)
+        self.lib.or_sheared_bake(C.byref(B), out.ctypes.data_as(_F), nthreads)
+        return out
+
+    def attach_sheared(self, S, B, slices):
+        """Makes S read the baked slices (turb_model 1)."""
+        slices = np.ascontiguousarray(slices, dtype=np.float32)
+        S.turb_model = 1
+        S.ts = B
+        S.ts_slices = slices.ctypes.data_as(_F)
+        S._ts_keep = slices
+
+    def gnoise(self, x, y, z, seed):
+        return self.lib.or_sheared_gnoise(x, y, z, seed)
+
+    def fbm(self, x, y, z, octaves, seed):
+        return self.lib.or_sheared_fbm(x, y, z, octaves, seed)
+
+    def sheared_noise(self, B, R, phi, z, t):
+        return self.lib.or_sheared_noise(C.byref(B), R, phi, z, t)
+
+    def sheared_tex(self, T, x, y, z):
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        nz, ny, nx = T.shape
+        return self.lib.or_sheared_tex(T.ctypes.data_as(_F), nx, ny, nz, x, y, z)
+
+    def sheared_cell(self, B, i, j, l, k):
+        return self.lib.or_sheared_cell(C.byref(B), i, j, l, k)
+
+    def sheared_factor(self, S, r, R, z, phi, t):
+        return self.lib.or_sheared_factor(C.byref(S), r, R, z, phi, t)
 
     # --- radio post step and tone map (post.cpp) -----------------------------------------------
     def post_blur(self, img, sigma):
@@ -365,7 +431,7 @@ class Ref:
         return self.lib.ref_vacuum_fate(a, _dp(p), tol, r_escape, max_steps)
 
     def build_scene(self, path, sets=""):
-        out = np.zeros(29)
+        out = np.zeros(31)
         k2 = np.zeros(512, dtype=np.float32)
         ok = self.lib.ref_build_scene(path.encode(), sets.encode(), _dp(out), k2.ctypes.data_as(_F))
         if not ok:

@@ -98,7 +98,7 @@ int ref_build_scene(const char* path, const char* sets, double* out, float* k2) 
                   (double)S.mode, (double)S.flow_on, S.n0, S.n_index, S.theta_index, S.theta0, S.h_r, S.beta_p,
                   S.r_flow_max, S.nu_obs, S.rg_cm, S.flow_kep, (double)S.flow_turb_amp, (double)S.k2_n,
                   (double)S.k2_l0, (double)S.k2_dl, (double)S.seed, I.distance_cm, I.uas_per_m,
-                  I.radio_flux_target_jy, I.mass_msun};
+                  I.radio_flux_target_jy, I.mass_msun, (double)S.flow_turb_scale, (double)S.flow_tcorr};
     std::memcpy(out, v, sizeof v);
     for (int i = 0; i < S.k2_n; ++i) k2[i] = T.k2_ln[i];
     return 1;

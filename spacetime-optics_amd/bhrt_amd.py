@@ -15,6 +15,7 @@ BH_OK, BH_ERR_ARG, BH_ERR_HIP, BH_ERR_NODEVICE, BH_ERR_UNSUPPORTED = 0, -1, -2, 
 MODE_VISIBLE, MODE_RADIO = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ROWS, ORDER_IMPACT = 1, 2
+TURB_GRID, TURB_SHEARED = 0, 1
 EMIS_EXACT, EMIS_MLP = 0, 1
 
 # Every symbol declared in include/bhrt.h and include/bhrt_host.h
@@ -24,7 +25,7 @@ EXPORTS = [
     "bhh_config_str", "bhh_build_scene", "bhh_turbulence", "bhh_build_shot", "bhh_make_frame",
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
-    "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap",
+    "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap", "bh_sheared_slices",
 ]
 MLP_NPARAM = 8706
 SHOT_VALUES = 20   # values per frame of bhh_build_shot
@@ -39,6 +40,7 @@ class Scene(C.Structure):
         ("n0", C.c_double), ("n_index", C.c_double), ("theta_index", C.c_double), ("theta0", C.c_double),
         ("h_r", C.c_double), ("beta_p", C.c_double), ("r_flow_max", C.c_double), ("nu_obs", C.c_double),
         ("rg_cm", C.c_double), ("flow_kep", C.c_double), ("flow_turb_amp", C.c_float), ("seed", C.c_uint32),
+        ("flow_turb_scale", C.c_float), ("flow_tcorr", C.c_float),
     ]
 
 
@@ -47,6 +49,7 @@ class Tables(C.Structure):
         ("k2_ln", C.POINTER(C.c_float)), ("k2_n", C.c_int), ("k2_l0", C.c_float), ("k2_dl", C.c_float),
         ("turb", C.POINTER(C.c_float)), ("turb_n", C.c_int), ("turb_nt", C.c_int),
         ("turb_half", C.c_double), ("turb_t0", C.c_double), ("turb_dt", C.c_double),
+        ("turb_model", C.c_int), ("ts_dims", C.c_int * 4),
     ]
 
 
@@ -185,6 +188,8 @@ def load():
     L.bh_autotune.restype = C.c_int
     L.bh_autotune.argtypes = [C.c_void_p, C.POINTER(BatchFrame), C.c_int, C.POINTER(TraceOpts), C.POINTER(BatchCfg),
                               _D, _D, C.c_int, C.POINTER(C.c_int)]
+    L.bh_sheared_slices.restype = C.c_int
+    L.bh_sheared_slices.argtypes = [C.c_void_p, C.POINTER(Frame), _F, C.c_size_t]
     L.bh_post_radio.restype = C.c_int
     L.bh_post_radio.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(PostParams), C.c_double, _D, _D]
     L.bh_tonemap.restype = C.c_int
@@ -357,7 +362,7 @@ class Tracer:
     """A device context (bh_ctx)."""
 
     def __init__(self, scene, k2=None, k2_l0=0.0, k2_dl=0.0, grid=None, turb_half=60.0, turb_t0=0.0, turb_dt=0.0,
-                 device=0, settings=None):
+                 device=0, settings=None, turb_model=TURB_GRID, ts_dims=(0, 0, 0, 0)):
         L = load()
         T = Tables()
         self._keep = []
@@ -379,6 +384,9 @@ class Tracer:
             T.turb_half = turb_half
             T.turb_t0 = turb_t0
             T.turb_dt = turb_dt
+        T.turb_model = turb_model
+        for i in range(4):
+            T.ts_dims[i] = int(ts_dims[i])
         rs = settings or Settings()
         ctx = C.c_void_p()
         err = C.create_string_buffer(512)
@@ -479,6 +487,15 @@ class Tracer:
         self._check(self.L.bh_vacuum_fates(self.ctx, ph.ctypes.data_as(_D), len(ph), tol, r_escape, max_steps,
                                            fates.ctypes.data_as(C.POINTER(C.c_int))), "bh_vacuum_fates")
         return fates
+
+    def sheared_slices(self, F, dims):
+        """The baked sheared-turbulence slices for frame F: (nt, nz, nlr, nphi) float32; dims as
+        given to the constructor (nphi, nlr, nz, nt)."""
+        nphi, nlr, nz, nt = dims
+        out = np.zeros((nt, nz, nlr, nphi), dtype=np.float32)
+        self._check(self.L.bh_sheared_slices(self.ctx, C.byref(F), out.ctypes.data_as(_F), out.size),
+                    "bh_sheared_slices")
+        return out
 
     # --- radio post step and display ---------------------------------------------------------
     def post_radio(self, img, blur_sigma_px=0.0, hide_direct=0.0, percentile=0.995):

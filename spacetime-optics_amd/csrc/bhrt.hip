@@ -51,7 +51,7 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-template <int PREC, bool RADIO, bool RAYS>
+template <int PREC, int RADIO, bool RAYS>
 __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, const bhm32::MixedParams M,
                                                  float4* __restrict__ px_out, bh_ray_out* __restrict__ rays_out,
                                                  BlockOut* __restrict__ blk, int per_xcd) {
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
 
 // WAVES > 1 asks the compiler for at least that many resident waves per SIMD (fewer VGPRs, possibly
 // spills); 1 leaves the register budget to the compiler.
-template <int PREC, bool RADIO, int WAVES, class Work>
+template <int PREC, int RADIO, int WAVES, class Work>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_persist(
     const Work wk, unsigned int* __restrict__ queue, int interval, int refill_min, int prio_age,
     BlockOut* __restrict__ blk) {
@@ -339,6 +339,21 @@ __global__ void k_zero_u32(unsigned int* __restrict__ p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// Bake of the sheared turbulence slices (render.cu:449-460): one work-item per cell, phi fastest.
+__global__ __launch_bounds__(256) void k_bake_sheared(const bhd::ShearedBake B, float* __restrict__ out,
+                                                      unsigned long long total) {
+    const unsigned long long idx = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    unsigned long long rest = idx;
+    const int i = (int)(rest % (unsigned)B.nphi);
+    rest /= (unsigned)B.nphi;
+    const int j = (int)(rest % (unsigned)B.nlr);
+    rest /= (unsigned)B.nlr;
+    const int l = (int)(rest % (unsigned)B.nz);
+    const int k = (int)(rest / (unsigned)B.nz);
+    out[idx] = bhd::sh_cell(B, i, j, l, k);
+}
+
 /* Queue order of a persistent launch: the tiles of each frame sorted by the impact parameter b of
    their centre ray, smallest first. b^2 = eta + lam^2 (+ a^2 cos^2 theta_cam, the same for the whole
    frame) is the distance from the hole's image for a distant camera; rays near and inside the
@@ -508,6 +523,11 @@ TraceParams make_params_scene(const bh_ctx* c, const bh_scene& S, const bh_frame
     P.turb_inv_dx = c->turb_n > 0 ? c->turb_n / (2.0 * c->turb_half) : 0.0;
     P.turb_t0 = c->turb_t0;
     P.turb_inv_dt = c->turb_dt > 0 ? 1.0 / c->turb_dt : 0.0;
+    if (c->turb_model == BH_TURB_SHEARED) {
+        // the slices baked for this scene and camera (prepare_turbulence runs before every launch)
+        P.turb_on = (S.flow_turb_amp > 0.f && c->ts_valid) ? 1 : 0;
+        P.ts = c->d_tsv;
+    }
     if (f) {
         P.ph0 = f->ph;
         P.t0 = f->t;
@@ -530,6 +550,63 @@ TraceParams make_params_scene(const bh_ctx* c, const bh_scene& S, const bh_frame
 
 TraceParams make_params(const bh_ctx* c, const bh_frame* f) { return make_params_scene(c, c->S, f); }
 
+// Sheared turbulence of a radio scene seen from frame f: bakes the slices (render.cu:594-619,
+// 679-682) unless the ones in memory belong to the same noise parameters and camera.
+int prepare_turbulence(bh_ctx* c, const bh_scene& S, const bh_frame* f) {
+    if (c->turb_model != BH_TURB_SHEARED || !f || S.mode != BH_MODE_RADIO || !S.flow_on || !(S.flow_turb_amp > 0.f))
+        return BH_OK;
+    if (!(S.flow_turb_scale > 0.f) || !(S.flow_tcorr > 0.f))
+        return fail(c, BH_ERR_ARG, "sheared turbulence needs flow_turb_scale > 0 and flow_tcorr > 0");
+    if (!(f->r > 0.0)) return fail(c, BH_ERR_ARG, "sheared turbulence needs the camera radius (frame r > 0)");
+    bhd::ShearedBake B{};
+    B.a = S.a;
+    B.r_isco = S.r_isco;
+    B.cam_t = f->t;
+    B.cam_r = f->r;
+    B.lr0 = (float)std::log(0.8 * S.rh);
+    B.lr1 = (float)std::log(S.r_flow_max * 1.05);
+    B.zmax = (float)std::min(2.0, 5.0 * S.h_r);
+    B.nphi = c->ts_dims[0];
+    B.nlr = c->ts_dims[1];
+    B.nz = c->ts_dims[2];
+    B.nt = c->ts_dims[3];
+    B.d0 = -250.f;
+    B.dd = 260.f / (float)(B.nt - 1);
+    B.amp = S.flow_turb_amp;
+    B.scale = S.flow_turb_scale;
+    B.tcorr = S.flow_tcorr;
+    B.seed = S.seed ^ 0x51f15eedu;
+    const std::array<double, 16> key = {B.a, B.r_isco, B.cam_t, B.cam_r, S.rh, S.r_flow_max, S.h_r, (double)B.amp,
+                                        (double)B.scale, (double)B.tcorr, (double)B.seed, (double)B.nphi,
+                                        (double)B.nlr, (double)B.nz, (double)B.nt, 1.0};
+    if (c->ts_valid && key == c->ts_key) return BH_OK;
+    const unsigned long long total = (unsigned long long)B.nphi * B.nlr * B.nz * B.nt;
+    c->ts_valid = false;
+    int rc = ensure(c, &c->d_ts, &c->ts_cap, (size_t)total);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_tsv, &c->tsv_cap, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bake_sheared, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->st, B, c->d_ts, total);
+    HIPCK(c, hipGetLastError());
+    bhd::ShearedView& V = c->tsv_host;   // lives in the context until the copy has run
+    V.data = c->d_ts;
+    V.nphi = B.nphi;
+    V.nlr = B.nlr;
+    V.nz = B.nz;
+    V.nt = B.nt;
+    V.lr0 = B.lr0;
+    V.lr1 = B.lr1;
+    V.zmax = B.zmax;
+    V.d0 = B.d0;
+    V.dd = B.dd;
+    V.cam_t = B.cam_t;
+    V.cam_r = B.cam_r;
+    HIPCK(c, hipMemcpyAsync(c->d_tsv, &V, sizeof V, hipMemcpyHostToDevice, c->st));
+    c->ts_key = key;
+    c->ts_valid = true;
+    return BH_OK;
+}
+
 int check_frame(bh_ctx* c, const bh_frame* f) {
     if (!f) return fail(c, BH_ERR_ARG, "frame is NULL");
     if (f->W <= 0 || f->H <= 0 || f->W > 65536 || f->H > 65536) return fail(c, BH_ERR_ARG, "bad frame size");
@@ -540,13 +617,24 @@ int check_frame(bh_ctx* c, const bh_frame* f) {
 
 using KernelFn = void (*)(const TraceParams, const bhm32::MixedParams, float4*, bh_ray_out*, BlockOut*, int);
 
-KernelFn pick_kernel(int prec, bool radio, bool rays) {
-    if (prec == BH_PREC_FP64) {
-        if (radio) return rays ? k_trace<BH_PREC_FP64, true, true> : k_trace<BH_PREC_FP64, true, false>;
-        return rays ? k_trace<BH_PREC_FP64, false, true> : k_trace<BH_PREC_FP64, false, false>;
+// The tracers' RADIO argument for a scene mode and the context's turbulence model
+int radio_kind(const bh_ctx* c, int mode) {
+    if (mode != BH_MODE_RADIO) return bhd::RADIO_OFF;
+    return c->turb_model == BH_TURB_SHEARED ? bhd::RADIO_SHEARED : bhd::RADIO_GRID;
+}
+
+template <int PREC, bool RAYS>
+KernelFn pick_kernel_p(int radio) {
+    switch (radio) {
+        case bhd::RADIO_SHEARED: return k_trace<PREC, bhd::RADIO_SHEARED, RAYS>;
+        case bhd::RADIO_GRID: return k_trace<PREC, bhd::RADIO_GRID, RAYS>;
+        default: return k_trace<PREC, bhd::RADIO_OFF, RAYS>;
     }
-    if (radio) return rays ? k_trace<BH_PREC_MIXED, true, true> : k_trace<BH_PREC_MIXED, true, false>;
-    return rays ? k_trace<BH_PREC_MIXED, false, true> : k_trace<BH_PREC_MIXED, false, false>;
+}
+
+KernelFn pick_kernel(int prec, int radio, bool rays) {
+    if (prec == BH_PREC_FP64) return rays ? pick_kernel_p<BH_PREC_FP64, true>(radio) : pick_kernel_p<BH_PREC_FP64, false>(radio);
+    return rays ? pick_kernel_p<BH_PREC_MIXED, true>(radio) : pick_kernel_p<BH_PREC_MIXED, false>(radio);
 }
 
 constexpr double C_C = 2.99792458e10, C_H = 6.62607015e-27, C_ME = 9.1093837e-28, C_MP = 1.67262192e-24,
@@ -691,6 +779,8 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
         M.use_surrogate = 1;
         M.S = c->stab;
     }
+    rc = prepare_turbulence(c, c->S, f);
+    if (rc) return rc;
     TraceParams P = make_params(c, f);
     P.x0 = x0;
     P.y0 = y0;
@@ -707,7 +797,7 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     if (rays) rc = ensure(c, &c->d_rays, &c->rays_cap, (size_t)w * h);
     else rc = ensure(c, &c->d_px, &c->px_cap, (size_t)w * h);
     if (rc) return rc;
-    KernelFn k = pick_kernel(o.precision, c->S.mode == BH_MODE_RADIO, rays);
+    KernelFn k = pick_kernel(o.precision, radio_kind(c, c->S.mode), rays);
     HIPCK(c, hipEventRecord(c->ev0, c->st));
     hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, M, c->d_px, c->d_rays, c->d_blk, per_xcd);
     HIPCK(c, hipGetLastError());
@@ -734,27 +824,36 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
 
 using PersistFn = void (*)(const bhp::BatchWork, unsigned int*, int, int, int, BlockOut*);
 
+template <class Work>
+using WorkFn = void (*)(const Work, unsigned int*, int, int, int, BlockOut*);
+
+template <int PREC, int W, class Work>
+WorkFn<Work> pick_persist_p(int radio) {
+    switch (radio) {
+        case bhd::RADIO_SHEARED: return k_persist<PREC, bhd::RADIO_SHEARED, W, Work>;
+        case bhd::RADIO_GRID: return k_persist<PREC, bhd::RADIO_GRID, W, Work>;
+        default: return k_persist<PREC, bhd::RADIO_OFF, W, Work>;
+    }
+}
+
+template <int W, class Work>
+WorkFn<Work> pick_persist_pw(int prec, int radio) {
+    return prec == BH_PREC_FP64 ? pick_persist_p<BH_PREC_FP64, W, Work>(radio) : pick_persist_p<BH_PREC_MIXED, W, Work>(radio);
+}
+
 template <int W>
-PersistFn pick_persist_w(int prec, bool radio) {
-    using BW = bhp::BatchWork;
-    if (prec == BH_PREC_FP64)
-        return radio ? k_persist<BH_PREC_FP64, true, W, BW> : k_persist<BH_PREC_FP64, false, W, BW>;
-    return radio ? k_persist<BH_PREC_MIXED, true, W, BW> : k_persist<BH_PREC_MIXED, false, W, BW>;
+PersistFn pick_persist_w(int prec, int radio) {
+    return pick_persist_pw<W, bhp::BatchWork>(prec, radio);
 }
 
 // adaptive passes: the kernel compiled for at least 3 waves per SIMD (waves 3 or 4) or for the
 // compiler's register budget (waves 0)
-template <int W, class Work>
-void (*pick_adaptive_w(int prec, bool radio))(const Work, unsigned int*, int, int, int, BlockOut*) {
-    if (prec == BH_PREC_FP64) return radio ? k_persist<BH_PREC_FP64, true, W, Work> : k_persist<BH_PREC_FP64, false, W, Work>;
-    return radio ? k_persist<BH_PREC_MIXED, true, W, Work> : k_persist<BH_PREC_MIXED, false, W, Work>;
-}
 template <class Work>
-void (*pick_adaptive(int prec, bool radio, int waves))(const Work, unsigned int*, int, int, int, BlockOut*) {
-    return waves >= 3 ? pick_adaptive_w<3, Work>(prec, radio) : pick_adaptive_w<1, Work>(prec, radio);
+WorkFn<Work> pick_adaptive(int prec, int radio, int waves) {
+    return waves >= 3 ? pick_persist_pw<3, Work>(prec, radio) : pick_persist_pw<1, Work>(prec, radio);
 }
 
-PersistFn pick_persist(int prec, bool radio, int waves) {
+PersistFn pick_persist(int prec, int radio, int waves) {
     switch (waves) {
         case 3: return pick_persist_w<3>(prec, radio);
         case 4: return pick_persist_w<4>(prec, radio);
@@ -828,6 +927,22 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     rc = resolve_cfg(c, o.precision, &cfg);
     if (rc) return rc;
     const int mode = frames[0].scene.mode;
+    if (c->turb_model == BH_TURB_SHEARED) {
+        // one set of slices per launch: every frame must see the same turbulence bake
+        for (int i = 1; i < n; ++i) {
+            const bh_scene& A = frames[0].scene;
+            const bh_scene& Bs = frames[i].scene;
+            if (frames[i].frame.t != frames[0].frame.t || frames[i].frame.r != frames[0].frame.r || Bs.a != A.a ||
+                Bs.flow_turb_amp != A.flow_turb_amp || Bs.flow_turb_scale != A.flow_turb_scale ||
+                Bs.flow_tcorr != A.flow_tcorr || Bs.seed != A.seed || Bs.h_r != A.h_r || Bs.rh != A.rh ||
+                Bs.r_flow_max != A.r_flow_max)
+                return fail(c, BH_ERR_UNSUPPORTED,
+                            "sheared turbulence: the frames of a batch must share the camera time and radius and "
+                            "the scene's noise parameters (one bake per launch)");
+        }
+        rc = prepare_turbulence(c, frames[0].scene, &frames[0].frame);
+        if (rc) return rc;
+    }
     long long total = 0;
     std::vector<bhp::BatchFrame> hb((size_t)n);
     for (int i = 0; i < n; ++i) {
@@ -886,7 +1001,7 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid);
     if (rc) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_bframes, hb.data(), sizeof(bhp::BatchFrame) * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PersistFn k = pick_persist(o.precision, mode == BH_MODE_RADIO, cfg.waves);
+    PersistFn k = pick_persist(o.precision, radio_kind(c, mode), cfg.waves);
     const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
@@ -974,6 +1089,8 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     bh_batch_cfg cfg = c->tuned_ok ? c->tuned : batch_defaults(o.precision);
     rc = resolve_cfg(c, o.precision, &cfg);
     if (rc) return rc;
+    rc = prepare_turbulence(c, c->S, f);
+    if (rc) return rc;
     const int grid = c->n_cu * cfg.grid_per_cu;
     const int prio = cfg.prio_age > 0 ? cfg.prio_age : 0;
     const bool ordered = cfg.order == BH_ORDER_IMPACT;
@@ -1014,12 +1131,13 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
         return rc;
     const bhp::BatchFrame* dB = (const bhp::BatchFrame*)c->d_bframes;
     const bool radio = c->S.mode == BH_MODE_RADIO;
+    const int rk = radio_kind(c, c->S.mode);
     HIPCK(c, hipEventRecord(c->ev0, c->st));
     HIPCK(c, hipMemcpyAsync(c->d_bframes, B, sizeof B, hipMemcpyHostToDevice, c->st));
     // pass 1
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
     if (ordered) enqueue_tile_order(c->st, dB + 1, 1, (unsigned int)tiles, c->d_ohist);
-    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, radio, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
+    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, rk, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
                        bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, prio,
                        c->d_blk);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
@@ -1055,7 +1173,7 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
             HIPCK(c, hipMemcpyAsync(c->d_bframes + sizeof(bhp::BatchFrame), &B[1], sizeof B[1], hipMemcpyHostToDevice,
                                     c->st));
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
-            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, radio, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
+            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, rk, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
                                bhp::ChunkWork{dB + 1, 1, spp, npix, c->d_offsets, c->d_samp2}, c->d_queue,
                                cfg.interval, cfg.refill_min, prio, c->d_blk);
             hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
@@ -1153,7 +1271,21 @@ int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const 
                 return bail(BH_ERR_HIP);
             }
         }
-        if (tables->turb && tables->turb_n >= 2) {
+        if (tables->turb_model != BH_TURB_GRID && tables->turb_model != BH_TURB_SHEARED) {
+            c->err = "turb_model must be BH_TURB_GRID or BH_TURB_SHEARED";
+            return bail(BH_ERR_ARG);
+        }
+        if (tables->turb_model == BH_TURB_SHEARED) {
+            c->turb_model = BH_TURB_SHEARED;
+            const int def[4] = {512, 192, 48, 24};   // render.cu:448
+            for (int i = 0; i < 4; ++i) c->ts_dims[i] = tables->ts_dims[i] > 0 ? tables->ts_dims[i] : def[i];
+            const long long cells = (long long)c->ts_dims[0] * c->ts_dims[1] * c->ts_dims[2] * c->ts_dims[3];
+            if (c->ts_dims[0] < 2 || c->ts_dims[1] < 2 || c->ts_dims[2] < 2 || c->ts_dims[3] < 2 ||
+                cells > (1ll << 31)) {
+                c->err = "sheared turbulence dimensions must be >= 2 with at most 2^31 cells";
+                return bail(BH_ERR_ARG);
+            }
+        } else if (tables->turb && tables->turb_n >= 2) {
             int nt = tables->turb_nt > 0 ? tables->turb_nt : 1;
             if (nt > 1 && !(tables->turb_dt > 0.0)) {
                 c->err = "turbulence time slices need turb_dt > 0";
@@ -1246,6 +1378,23 @@ int bh_trace_tile(bh_ctx* c, const bh_frame* f, int x0, int y0, int w, int h, co
     int rc = launch_trace(c, f, opts, x0, y0, w, h, true, nullptr);
     if (rc) return rc;
     HIPCK(c, hipMemcpyAsync(out, c->d_rays, sizeof(bh_ray_out) * (size_t)w * h, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipStreamSynchronize(c->st));
+    return BH_OK;
+}
+
+int bh_sheared_slices(bh_ctx* c, const bh_frame* f, float* out, size_t n) {
+    if (!c || !f || !out) return BH_ERR_ARG;
+    HIPCK(c, hipSetDevice(c->device));
+    if (c->turb_model != BH_TURB_SHEARED) return fail(c, BH_ERR_ARG, "the context has no sheared turbulence");
+    const size_t total = (size_t)c->ts_dims[0] * c->ts_dims[1] * c->ts_dims[2] * c->ts_dims[3];
+    if (n < total) return fail(c, BH_ERR_ARG, "output too small for the turbulence slices");
+    bh_scene S = c->S;
+    S.mode = BH_MODE_RADIO;
+    S.flow_on = 1;
+    if (!(S.flow_turb_amp > 0.f)) return fail(c, BH_ERR_ARG, "the scene has no turbulence (flow_turb_amp = 0)");
+    int rc = prepare_turbulence(c, S, f);
+    if (rc) return rc;
+    HIPCK(c, hipMemcpyAsync(out, c->d_ts, sizeof(float) * total, hipMemcpyDeviceToHost, c->st));
     HIPCK(c, hipStreamSynchronize(c->st));
     return BH_OK;
 }
@@ -1410,7 +1559,7 @@ void bh_destroy(bh_ctx* c) {
                     c->d_fates,  c->d_mlp,    c->d_adam_m, c->d_adam_v,  c->d_stab,   c->d_sbad,   c->d_bframes,
                     c->d_queue,  c->d_bout,   c->d_samp1,  c->d_acc,     c->d_ystat,  c->d_extra,  c->d_chunks,
                     c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2,  c->d_order,  c->d_ohist,
-                    c->d_post_lev, c->d_post_tmp, c->d_post_wt, c->d_post_rgb};
+                    c->d_post_lev, c->d_post_tmp, c->d_post_wt, c->d_post_rgb, c->d_ts, c->d_tsv};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -9,6 +9,7 @@
 
 #include "../../include/bhrt.h"
 #include "mlp.h"
+#include "trace_fp64.h"
 
 constexpr int NCNT = 5;   // counters per block: attempts, accepted, flow, limited, fp64 attempts
 
@@ -32,6 +33,16 @@ struct bh_ctx {
     float* d_turb = nullptr;
     int turb_n = 0, turb_nt = 0;
     double turb_half = 0, turb_t0 = 0, turb_dt = 0;
+    // sheared turbulence: slices baked on the device for the current scene and camera
+    int turb_model = BH_TURB_GRID;
+    int ts_dims[4] = {512, 192, 48, 24};
+    float* d_ts = nullptr;
+    size_t ts_cap = 0;
+    bool ts_valid = false;
+    std::array<double, 16> ts_key{};
+    bhd::ShearedView* d_tsv = nullptr;   // the view the kernels read (TraceParams.ts)
+    size_t tsv_cap = 0;
+    bhd::ShearedView tsv_host{};
     float4* d_px = nullptr;
     size_t px_cap = 0;
     bh_ray_out* d_rays = nullptr;

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 #include <thread>
 
 namespace bh {
@@ -20,6 +21,8 @@ bh_tables SceneTables::view() const {
     t.turb_half = turb_half;
     t.turb_t0 = turb_t0;
     t.turb_dt = turb_dt;
+    t.turb_model = turb_model;
+    for (int i = 0; i < 4; ++i) t.ts_dims[i] = ts_dims[i];
     return t;
 }
 
@@ -175,14 +178,32 @@ bool build_scene(const Config& c, bh_scene* S, SceneTables* T, SceneInfo* info, 
     S->rg_cm = info->rg_cm;
     S->flow_kep = c.num("flow.keplerian_fraction", 0.9);
     S->flow_turb_amp = (float)c.num("flow.turbulence", 0.7);
+    S->flow_turb_scale = (float)c.num("flow.turbulence_scale", 0.45);
+    S->flow_tcorr = (float)c.num("flow.correlation_orbits", 0.5);
     info->radio_flux_target_jy = c.num("flow.flux_jy", 0.0);
     info->normalize_fov_m = c.num("flow.normalize_fov_m", 40.0);
     k2_table(512, &T->k2_ln, &T->k2_l0, &T->k2_dl);
 
-    // Turbulence grid (new [turbulence] keys); amplitude from the reference key flow.turbulence
+    // Turbulence (new [turbulence] keys; amplitude from the reference key flow.turbulence):
+    // model = sheared (default: the reference's baked sheared noise, baked by the library per frame;
+    // cells = nphi,nlnR,nz and slices override its 512,192,48 x 24) or grid (a seeded log-normal
+    // Cartesian grid made here)
     T->turb.clear();
     T->turb_n = T->turb_nt = 0;
-    if (with_grid && S->mode == BH_MODE_RADIO && S->flow_on && S->flow_turb_amp > 0.f) {
+    T->turb_model = BH_TURB_GRID;
+    for (int i = 0; i < 4; ++i) T->ts_dims[i] = 0;
+    const std::string model = c.str("turbulence.model", "sheared");
+    if (model != "sheared" && model != "grid") {
+        if (err) *err = "turbulence.model must be sheared or grid";
+        return false;
+    }
+    if (model == "sheared") {
+        T->turb_model = BH_TURB_SHEARED;
+        std::stringstream ss(c.str("turbulence.cells", ""));
+        std::string item;
+        for (int i = 0; i < 3 && std::getline(ss, item, ','); ++i) T->ts_dims[i] = std::atoi(item.c_str());
+        T->ts_dims[3] = c.integer("turbulence.slices", 0);
+    } else if (with_grid && S->mode == BH_MODE_RADIO && S->flow_on && S->flow_turb_amp > 0.f) {
         int n = c.integer("turbulence.grid", 128);
         int nt = std::max(1, c.integer("turbulence.slices", 1));
         if (n < 2 || n > 1024) {

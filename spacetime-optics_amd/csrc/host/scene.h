@@ -35,6 +35,8 @@ struct SceneTables {
     std::vector<float> turb;
     int turb_n = 0, turb_nt = 0;
     double turb_half = 0, turb_t0 = 0, turb_dt = 0;
+    int turb_model = BH_TURB_GRID;
+    int ts_dims[4] = {0, 0, 0, 0};
     bh_tables view() const;   // non-owning C view
 };
 

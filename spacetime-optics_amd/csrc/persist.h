@@ -113,10 +113,10 @@ struct ChunkWork {
     }
 };
 
-template <int PREC, bool RADIO>
+template <int PREC, int RADIO>
 struct Tracer;
 
-template <bool RADIO>
+template <int RADIO>
 struct Tracer<BH_PREC_FP64, RADIO> {
     using State = bhd::RayState;
     static __device__ __forceinline__ void init(const TraceParams& P, const bhm32::MixedParams&, double x, double y,
@@ -132,7 +132,7 @@ struct Tracer<BH_PREC_FP64, RADIO> {
     }
 };
 
-template <bool RADIO>
+template <int RADIO>
 struct Tracer<BH_PREC_MIXED, RADIO> {
     using State = bhm32::RayStateM;
     static __device__ __forceinline__ void init(const TraceParams& P, const bhm32::MixedParams& M, double x, double y,
@@ -147,7 +147,7 @@ struct Tracer<BH_PREC_MIXED, RADIO> {
     }
 };
 
-template <int PREC, bool RADIO, class Work>
+template <int PREC, int RADIO, class Work>
 __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __restrict__ queue, int interval,
                                              int refill_min, int prio_age, double* flux_out,
                                              unsigned long long* cnt_out) {

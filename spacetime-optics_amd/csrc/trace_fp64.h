@@ -25,6 +25,11 @@ constexpr double C_C = 2.99792458e10, C_KB = 1.380649e-16, C_H = 6.62607015e-27,
                  C_MP = 1.67262192e-24, C_QE = 4.80320471e-10;
 constexpr double PI = 3.14159265358979323846;
 
+// The tracers' RADIO template argument: no flow (thermal disk), or the radio flow with the grid
+// turbulence or with the sheared turbulence. The turbulence model is a compile-time choice so a
+// kernel carries the lookup code of one model only (registers and instruction cache).
+constexpr int RADIO_OFF = 0, RADIO_GRID = 1, RADIO_SHEARED = 2;
+
 // Dormand-Prince 1980 tableau (kerr.hpp:124-129): the same constant expressions, folded by the compiler.
 constexpr double a21 = 1.0 / 5, a31 = 3.0 / 40, a32 = 9.0 / 40, a41 = 44.0 / 45, a42 = -56.0 / 15, a43 = 32.0 / 9;
 constexpr double a51 = 19372.0 / 6561, a52 = -25360.0 / 2187, a53 = 64448.0 / 6561, a54 = -212.0 / 729;
@@ -32,6 +37,15 @@ constexpr double a61 = 9017.0 / 3168, a62 = -355.0 / 33, a63 = 46732.0 / 5247, a
 constexpr double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192, b5 = -2187.0 / 6784, b6 = 11.0 / 84;
 constexpr double e1 = 71.0 / 57600, e3 = -71.0 / 16695, e4 = 71.0 / 1920, e5 = -17253.0 / 339200, e6 = 22.0 / 525,
                  e7 = -1.0 / 40;
+
+// Baked sheared-turbulence slices (turb_sheared.h): nt slices of nz x nlr x nphi floats, phi fastest,
+// with the slice coordinates and the camera they were baked for.
+struct ShearedView {
+    const float* data;
+    int nphi, nlr, nz, nt;
+    float lr0, lr1, zmax, d0, dd;
+    double cam_t, cam_r;
+};
 
 // Uniform scene and frame parameters: passed by value as the kernel argument, so they sit in SGPRs.
 struct TraceParams {
@@ -45,6 +59,9 @@ struct TraceParams {
     const float* __restrict__ turb;
     int turb_n, turb_nt;
     double turb_half, turb_inv_dx, turb_t0, turb_inv_dt;
+    // sheared turbulence (RADIO_SHEARED kernels): the slices and their coordinates, in device memory
+    // so the kernel arguments stay small
+    const struct ShearedView* __restrict__ ts;
     // frame (bh_frame)
     double ph0, t0;
     double kp[11];
@@ -74,6 +91,9 @@ __device__ __forceinline__ uint32_t pcg(uint32_t v) {
     uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
     return (w >> 22u) ^ w;
 }
+// the reference's sheared turbulence (bake and lookup), in this namespace
+#include "turb_sheared.h"
+
 __device__ __forceinline__ void sample_offsets(const TraceParams& P, int px, int py, int s, double* tx, double* ty) {
     float jx = 0.5f, jy = 0.5f;
     if (P.jitter) {
@@ -258,6 +278,7 @@ __device__ __forceinline__ double turb_factor(const TraceParams& P, double r, do
 }
 
 // blackholes/src/emission.cuh:99-175 (exact thermal-synchrotron coefficients)
+template <int RADIO>
 __device__ __forceinline__ void flow_step(const TraceParams& P, double r, double u, double phi, double t, double lam,
                                           double Ecam, double h, double* dI, double* dtau) {
     *dI = 0.0;
@@ -271,7 +292,10 @@ __device__ __forceinline__ void flow_step(const TraceParams& P, double r, double
     if (vert < 1e-7) return;
     double lr = bhm::log(r);
     double ne = P.n0 * bhm::exp(-P.n_index * lr) * vert;
-    if (P.turb_on) ne *= turb_factor(P, r, u, phi, t);
+    if (P.turb_on) {
+        if constexpr (RADIO == RADIO_SHEARED) ne *= (double)sheared_factor(P, r, R, z, phi, t);
+        else ne *= turb_factor(P, r, u, phi, t);
+    }
     ne *= 1.0 - bhm::exp(-(r - P.rh) * (r - P.rh) / 0.25);
     if (ne < 1e-30) return;
     double Th = P.theta0 * bhm::exp(-P.theta_index * lr);
@@ -371,7 +395,7 @@ struct RayState {
 };
 
 // blackholes/src/trace.cuh:62-80: initial state of the ray through image-plane offset (x, yv)
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ void ray_init(const TraceParams& P, double x, double yv, RayState& R) {
     camera_photon(P, x, yv, R.c, R.s);
     {
@@ -395,7 +419,7 @@ __device__ __forceinline__ void ray_init(const TraceParams& P, double x, double 
 }
 
 // One iteration of trace.cuh:81-163 (one DP54 attempt); sets R.done when the ray stops.
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ void ray_step(const TraceParams& P, RayState& R) {
     const bool flow = RADIO && P.flow_on;
     State n;
@@ -430,7 +454,7 @@ __device__ __forceinline__ void ray_step(const TraceParams& P, RayState& R) {
     if (flow && n.y0 < P.r_flow_max) {
         double dI, dtau;
         ++R.flow_steps;
-        flow_step(P, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, &dI, &dtau);
+        flow_step<RADIO>(P, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, &dI, &dtau);
         double cI = R.trans * dI;
         int ord = (int)(R.psi * (1.0 / PI));
         R.acc0 += cI;
@@ -486,7 +510,7 @@ __device__ __forceinline__ void ray_result(const RayState& R, RayResult& out) {
 }
 
 // blackholes/src/trace.cuh:62-165 (disk and sky branches out of scope), FP64 intensity.
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ void trace_ray_fp64(const TraceParams& P, double x, double yv, RayResult& out) {
     RayState R;
     ray_init<RADIO>(P, x, yv, R);

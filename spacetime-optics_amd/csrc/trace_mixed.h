@@ -250,11 +250,31 @@ __device__ __forceinline__ float turb_factor_f32(const TraceParams& P, float r, 
     return A + (B - A) * f;
 }
 
+// The sheared turbulence in FP32 (same slices and filter as bhd::sheared_factor): slice coordinates
+// from FP32 logs, the emission time relative to the camera's taken in FP64 before it is rounded.
+__device__ __forceinline__ float sheared_factor_f32(const TraceParams& P, float r, float R, float z, double phid,
+                                                    double td) {
+    const bhd::ShearedView& T = *P.ts;
+    const float u = (lnf(R) - T.lr0) * rcpf(T.lr1 - T.lr0);
+    const float v = (float)(phid * (0.5 / bhd::PI));
+    const float w = 0.5f + 0.5f * (z * rcpf(R)) * rcpf(T.zmax);
+    const float cr = (float)T.cam_r;
+    const float dt = (float)(td - T.cam_t) + (cr - r + 2.f * lnf(cr * rcpf(r)));   // t - t_ref(r)
+    const float s = fminf(fmaxf((dt - T.d0) * rcpf(T.dd), 0.f), (float)(T.nt - 1) - 1e-4f);
+    const int k = (int)s;
+    const float f = s - (float)k;
+    const size_t slice = (size_t)T.nphi * T.nlr * T.nz;
+    const float A = bhd::sh_tex(T.data + (size_t)k * slice, T.nphi, T.nlr, T.nz, v, u, w);
+    const float B = bhd::sh_tex(T.data + (size_t)(k + 1) * slice, T.nphi, T.nlr, T.nz, v, u, w);
+    return A + (B - A) * f;
+}
+
 // Geometry + plasma of one flow sample; returns false when nothing is emitted or absorbed.
 struct FlowSample {
     float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl, ln_bnu;
 };
 
+template <int RADIO>
 __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisParams& E, double rd, double ud,
                                             double phid, double td, double lamd, double Ecamd, double hd,
                                             FlowSample& f) {
@@ -267,7 +287,10 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
     const float lr = lnf(r);
     float ln_ne = E.ln_n0 - E.n_index * lr + lnvert;
-    if (P.turb_on) ln_ne += lnf(turb_factor_f32(P, r, u, phid, td));
+    if (P.turb_on) {
+        if constexpr (RADIO == bhd::RADIO_SHEARED) ln_ne += lnf(sheared_factor_f32(P, r, R, z, phid, td));
+        else ln_ne += lnf(turb_factor_f32(P, r, u, phid, td));
+    }
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
     if (!(soft > 0.f)) return false;
@@ -372,7 +395,7 @@ struct RayStateM {
     bool done, k1_f32;
 };
 
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedParams& M, double x, double yv,
                                            RayStateM& R) {
     bhd::camera_photon(P, x, yv, R.c, R.s);
@@ -403,7 +426,7 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
 }
 
 // One attempt of trace.cuh:81-163 under the mixed policy.
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedParams& M, RayStateM& R) {
     const bool flow = RADIO && P.flow_on;
     State n;
@@ -450,7 +473,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     if (flow && n.y0 < M.r_cut) {
         ++R.flow_steps;
         FlowSample f;
-        if (flow_sample(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
+        if (flow_sample<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
             float ln_j, ln_a, fj, fa;
             if (M.use_surrogate &&
                 surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
@@ -522,7 +545,7 @@ __device__ __forceinline__ int ray_result_m(const RayStateM& R, RayResult& out) 
 
 /* blackholes/src/trace.cuh:62-165 under the mixed policy. Returns the number of FP64 step attempts
    (near field and flow); out.steps counts all attempts. */
-template <bool RADIO>
+template <int RADIO>
 __device__ __forceinline__ int trace_ray_mixed(const TraceParams& P, const MixedParams& M, double x, double yv,
                                                RayResult& out) {
     RayStateM R;

@@ -168,7 +168,8 @@ def main():
     s, _ = ref.build_scene(os.path.join(SHOTS, "c2_radio_1024.ini"),
                            "bh.spin=1.5;render.seed=7;render.horizon_margin=0.02;flow.n0=3e5")
     out["scene_set_out"] = s
-    s, _ = ref.build_scene(os.path.join(SHOTS, "c2_radio_1024.ini"), "bh.spin=-0.3")
+    s, _ = ref.build_scene(os.path.join(SHOTS, "c2_radio_1024.ini"),
+                           "bh.spin=-0.3;flow.turbulence_scale=0.3;flow.correlation_orbits=0.8")
     out["scene_set2_out"] = s
 
     # build_shot: the track fixture and the reference's zoom and EHT shots
