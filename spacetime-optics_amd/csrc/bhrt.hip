@@ -749,6 +749,7 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
     E.h_over_mec2 = (float)(H / (ME * C * C));
     E.ln_nu_obs = (float)std::log(S.nu_obs);
     E.ln_rg = (float)std::log(S.rg_cm);
+    E.k2_inv_dl = c->k2_dl > 0.f ? 1.f / c->k2_dl : 0.f;
     M.ln_e2_over_c = (float)std::log(QE * QE / C);
     M.ln_e2_over_mec = (float)std::log(QE * QE / (ME * C));
     return M;
@@ -1296,6 +1297,10 @@ int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const 
                 return bail(BH_ERR_ARG);
             }
             size_t n = (size_t)tables->turb_n * tables->turb_n * tables->turb_n * nt;
+            if (n >= (1ull << 31)) {   // the mixed tracer addresses the slices with 32-bit offsets
+                c->err = "turbulence grid must hold fewer than 2^31 cells";
+                return bail(BH_ERR_ARG);
+            }
             c->turb_n = tables->turb_n;
             c->turb_nt = nt;
             c->turb_half = tables->turb_half;

@@ -7,7 +7,11 @@
      r_near, near the hole) - there the reference's steps are tolerance-limited and the
      radiative-transfer quadrature depends on that step sequence; tol_mid from r_tol to r_far (the
      flow's edge in radio mode); tol_far beyond (radio: scaled with the pixel's size, 1e-4 on
-     0.04 M pixels; vacuum: 1e-6).
+     0.04 M pixels; vacuum: 1e-6). Outside r_tol the step-size controller also scales the next step
+     by r_start / r_end of the accepted one (the 1/r scaling of the Mino-time step far from the
+     hole): the reference's error-only controller lags behind it and rejected every other step on
+     the way out (CPU sweep: 114 -> 95 attempts per ray, rejections 18 -> 2.6, errors unchanged).
+     Inside r_tol the controller is the reference's.
    - Emission region: emission, absorption and the transfer step cap are evaluated below r_cut =
      r_tol; the rest of the flow is traced as vacuum with tol_mid = tol_far / 10 (or, with
      full_flow, the whole flow as the FP64 tracer with tol_mid = 1e-6).
@@ -41,12 +45,29 @@ using bhd::TraceParams;
 #pragma clang fp contract(on)
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+// hardware FP32 square root (1.5 ulp): the correctly rounded sqrtf is a 15-instruction sequence
+__device__ __forceinline__ float sqrtf_(float x) { return __builtin_amdgcn_sqrtf(x); }
+// v_rcp_f64 is good to 2^-24.4 relative on MI355X; one Newton step brings it to 2.2e-15 (measured
+// over 2^20 values spanning 1e-13..1e13)
 __device__ __forceinline__ double rcpd(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// atan2(y, x) for y >= 0, in [0, pi]: octant reduction and t * P(t^2) on [0, 1] with P(0) = 1
+// (least-squares fit to atan on Chebyshev nodes, absolute error <= 4.0e-7, relative <= 1.6e-6)
+__device__ __forceinline__ float atan2_pos(float y, float x) {
+    const float ax = fabsf(x), mx = fmaxf(ax, y), mn = fminf(ax, y);
+    const float t = mx > 0.f ? mn * rcpf(mx) : 0.f, t2 = t * t;
+    float q = 0.0073739392682909966f;
+    q = q * t2 - 0.03551968187093735f;
+    q = q * t2 + 0.08216753602027893f;
+    q = q * t2 - 0.13398787379264832f;
+    q = q * t2 + 0.19861853122711182f;
+    q = q * t2 - 0.3332539498806f;
+    float a = t + t * t2 * q;
+    a = y > ax ? 1.5707963267948966f - a : a;
+    return x < 0.f ? 3.141592653589793f - a : a;
 }
 template <typename R>
 __device__ __forceinline__ R rcp(R x);
@@ -153,16 +174,18 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     o.k4 = (double)k.d4;
     o.k5 = (double)k.d5;
     E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
-    // FP64 scaled max-norm (kerr.hpp:145-155)
-    const double sc0 = tol * (1e-6 + fmax(fabs(s.y0), fabs(o.y0)));
-    const double sc1 = tol * (1e-6 + fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
-    const double sc3 = tol * fmax(1.0, fmax(fabs(s.y3), fabs(o.y3)));
-    double m = fabs(h * (double)E0) * rcpd(sc0);
-    m = fmax(m, fabs(h * (double)E1) * rcpd(sc1));
-    m = fmax(m, fabs(h * (double)E2) * rcpd(tol));
-    m = fmax(m, fabs(h * (double)E3) * rcpd(sc3));
-    m = fmax(m, fabs(h * (double)E4) * rcpd(tol4));
-    return m;
+    // scaled max-norm (kerr.hpp:145-155) in FP32: the norm only decides acceptance and the next step
+    // size (itself computed in FP32), so 24 bits are ample; the scales are >= tol * 1e-6
+    const float tf = (float)tol, hf = (float)h;
+    const float sc0 = tf * (1e-6f + (float)fmax(fabs(s.y0), fabs(o.y0)));
+    const float sc1 = tf * (1e-6f + (float)fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
+    const float sc3 = tf * fmaxf(1.f, (float)fmax(fabs(s.y3), fabs(o.y3)));
+    float m = fabsf(hf * (float)E0) * rcpf(sc0);
+    m = fmaxf(m, fabsf(hf * (float)E1) * rcpf(sc1));
+    m = fmaxf(m, fabsf(hf * (float)E2) * rcpf(tf));
+    m = fmaxf(m, fabsf(hf * (float)E3) * rcpf(sc3));
+    m = fmaxf(m, fabsf(hf * (float)E4) * rcpf((float)tol4));
+    return (double)m;
 }
 
 template <typename R>
@@ -191,7 +214,18 @@ struct EmisParams {
     float h_over_mec2;    // h / (m_e c^2)  (x = h nu / (k T_e) = h nu / (m_e c^2 Theta))
     float ln_nu_obs;
     float ln_rg;
+    float k2_inv_dl;   // 1 / (K2 table spacing)
 };
+
+// the K2 table lookup of bhd::table_lerp with the spacing's reciprocal in place of the division
+__device__ __forceinline__ float table_lerp_f32(const float* __restrict__ t, int n, float x0, float inv_dx, float x) {
+    float f = (x - x0) * inv_dx;
+    f = fminf(fmaxf(f, 0.f), (float)(n - 2) + 0.9999f);
+    const int i = (int)f;
+    f -= (float)i;
+    const float t0 = t[i], t1 = t[i + 1];
+    return t0 + (t1 - t0) * f;
+}
 
 // Thermal synchrotron: ln j and ln alpha from ln n_e, ln Theta, ln nu_f, ln nu_c, ln sin(theta_B)
 // (Leung et al. 2011 with the K2 table, Kirchhoff alpha = j / B_nu).
@@ -201,8 +235,8 @@ __device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const Em
     const float ln_nus = -1.5040774f + ln_nuc + 2.f * ln_th + ln_sinb;   // ln(2/9) + ...
     const float lnX = ln_nu - ln_nus;
     const float X = expf_(lnX), x3 = expf_(lnX * (1.f / 3.f));
-    const float brk = __builtin_sqrtf(X) + 1.887748625363387f * __builtin_sqrtf(x3);
-    const float lnk2 = bhd::table_lerp(P.k2, P.k2_n, P.k2_l0, P.k2_dl, ln_th);
+    const float brk = sqrtf_(X) + 1.887748625363387f * sqrtf_(x3);
+    const float lnk2 = table_lerp_f32(P.k2, P.k2_n, P.k2_l0, E.k2_inv_dl, ln_th);
     *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + 2.f * lnf(brk) - x3;
     // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x))
     const float x = E.h_over_mec2 * expf_(ln_nu - ln_th);
@@ -223,8 +257,9 @@ __device__ __forceinline__ float grid_trilinear_f32(const TraceParams& P, const 
     const float gz = fminf(fmaxf((pz + half) * inv - 0.5f, 0.f), hi);
     const int ix = min((int)gx, n - 2), iy = min((int)gy, n - 2), iz = min((int)gz, n - 2);
     const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
-    const size_t sy = (size_t)n, sz = (size_t)n * n;
-    const float* b = G + ((size_t)iz * sz + (size_t)iy * sy + (size_t)ix);
+    // 32-bit offsets: bh_create keeps the grid (n^3 cells x nt slices) below 2^31 cells
+    const unsigned int sy = (unsigned int)n, sz = (unsigned int)(n * n);
+    const float* b = G + ((unsigned int)iz * sz + (unsigned int)iy * sy + (unsigned int)ix);
     const float v000 = b[0], v100 = b[1], v010 = b[sy], v110 = b[sy + 1];
     const float v001 = b[sz], v101 = b[sz + 1], v011 = b[sz + sy], v111 = b[sz + sy + 1];
     const float c00 = v000 + (v100 - v000) * fx, c10 = v010 + (v110 - v010) * fx;
@@ -234,7 +269,7 @@ __device__ __forceinline__ float grid_trilinear_f32(const TraceParams& P, const 
 }
 
 __device__ __forceinline__ float turb_factor_f32(const TraceParams& P, float r, float u, double phid, double td) {
-    const float R = r * __builtin_sqrtf(fmaxf(1.f - u * u, 1e-12f));
+    const float R = r * sqrtf_(fmaxf(1.f - u * u, 1e-12f));
     const double two_pi = 6.283185307179586;
     const float ph = (float)(phid - two_pi * floor(phid * (1.0 / two_pi)));
     float sp, cp;
@@ -244,9 +279,10 @@ __device__ __forceinline__ float turb_factor_f32(const TraceParams& P, float r, 
     const float st = fminf(fmaxf((float)((td - P.turb_t0) * P.turb_inv_dt), 0.f), (float)(P.turb_nt - 1));
     const int k = min((int)st, P.turb_nt - 2);
     const float f = st - (float)k;
-    const size_t slice = (size_t)P.turb_n * P.turb_n * P.turb_n;
-    const float A = grid_trilinear_f32(P, P.turb + (size_t)k * slice, px, py, pz);
-    const float B = grid_trilinear_f32(P, P.turb + (size_t)(k + 1) * slice, px, py, pz);
+    const unsigned int slice = (unsigned int)(P.turb_n * P.turb_n * P.turb_n);
+    const float* G = P.turb + (unsigned int)k * slice;
+    const float A = grid_trilinear_f32(P, G, px, py, pz);
+    const float B = grid_trilinear_f32(P, G + slice, px, py, pz);
     return A + (B - A) * f;
 }
 
@@ -279,7 +315,7 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
                                             double phid, double td, double lamd, double Ecamd, double hd,
                                             FlowSample& f) {
     const float r = (float)rd, u = (float)ud, lam = (float)lamd;
-    const float s2 = fmaxf(1.f - u * u, 1e-12f), s = __builtin_sqrtf(s2);
+    const float s2 = fmaxf(1.f - u * u, 1e-12f), s = sqrtf_(s2);
     const float R = r * s, z = r * u;
     if (r > E.r_flow_max || R < 1e-3f) return false;
     const float zh = z * rcpf(E.h_r * R);
@@ -306,8 +342,8 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     const float g_tt = -(1.f - 2.f * r * iS), g_tph = -2.f * a * r * s2 * iS, g_phph = A * s2 * iS;
     const float omz = 2.f * a * r * iA;
     const float Rk = fmaxf(R, E.r_isco);
-    float Om = E.flow_kep * rcpf(Rk * __builtin_sqrtf(Rk) + a);
-    const float vmax = __builtin_sqrtf(Sigma * Delta * iA) * __builtin_amdgcn_rsqf(g_phph) * 0.999f;
+    float Om = E.flow_kep * rcpf(Rk * sqrtf_(Rk) + a);
+    const float vmax = sqrtf_(Sigma * Delta * iA) * __builtin_amdgcn_rsqf(g_phph) * 0.999f;
     Om = omz + fmaxf(-vmax, fminf(vmax, Om - omz));
     const float norm = -(g_tt + 2.f * g_tph * Om + g_phph * Om * Om);
     if (!(norm > 0.f)) return false;
@@ -319,7 +355,7 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     const float u_ph = ut * (g_tph + g_phph * Om);
     const float kb = lam - kdotu * u_ph;
     const float bb = g_phph + u_ph * u_ph;
-    const float cosb = kb * rcpf(kdotu * __builtin_sqrtf(bb));
+    const float cosb = kb * rcpf(kdotu * sqrtf_(bb));
     const float sinb2 = fmaxf(1.f - cosb * cosb, 1e-6f);
     f.ln_ne = ln_ne;
     f.ln_th = ln_th;
@@ -416,7 +452,7 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
     R.ncross = R.accepted = R.flow_steps = R.it = R.fate = R.fp64_attempts = 0;
     R.ppx = R.ppy = R.ppz = 0.f;
     if (RADIO) {
-        float sp = __builtin_sqrtf(fmaxf(1.f - (float)(R.s.y2 * R.s.y2), 0.f));
+        float sp = sqrtf_(fmaxf(1.f - (float)(R.s.y2 * R.s.y2), 0.f));
         float ph = (float)R.s.y4;
         R.ppx = sp * __cosf(ph);
         R.ppy = sp * __sinf(ph);
@@ -460,11 +496,11 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     }
     ++R.accepted;
     if (RADIO) {
-        float sn = __builtin_sqrtf(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
+        float sn = sqrtf_(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
         float ph = (float)n.y4;
         float pnx = sn * __cosf(ph), pny = sn * __sinf(ph), pnz = (float)n.y2;
         float cx = R.ppy * pnz - R.ppz * pny, cy = R.ppz * pnx - R.ppx * pnz, cz = R.ppx * pny - R.ppy * pnx;
-        R.psi += atan2f(__builtin_sqrtf(cx * cx + cy * cy + cz * cz), R.ppx * pnx + R.ppy * pny + R.ppz * pnz);
+        R.psi += atan2_pos(sqrtf_(cx * cx + cy * cy + cz * cz), R.ppx * pnx + R.ppy * pny + R.ppz * pnz);
         R.ppx = pnx;
         R.ppy = pny;
         R.ppz = pnz;
@@ -500,9 +536,15 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             }
         }
     }
+    const double r0 = R.s.y0;
     R.s = n;
     bhd::project_radial(P, R.c, R.s);
-    R.h = hc * fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
+    double fac = fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
+    // Outside the scene-tolerance region the step follows the radial scale: there the Mino-time
+    // step that meets the tolerance goes as 1/r (dr/dlambda ~ r^2), and the error-only controller
+    // lags one step behind it - on the way out every other step was rejected.
+    if (r0 >= M.r_tol) fac *= r0 * rcpd(R.s.y0);
+    R.h = hc * fac;
     if (R.s.y0 < P.r_stop) {
         R.fate = -1;
         R.done = true;
