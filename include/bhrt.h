@@ -32,7 +32,7 @@ enum { BH_MODE_VISIBLE = 0, BH_MODE_RADIO = 1 };      /* scene.hpp:10 RenderMode
 enum { BH_PREC_FP64 = 0, BH_PREC_MIXED = 1 };         /* plain FP64 (parity) or mixed FP32/FP64 */
 enum { BH_EMIS_EXACT = 0, BH_EMIS_MLP = 1 };          /* synchrotron coefficients: exact or surrogate */
 enum { BH_FATE_LIMIT = 0, BH_FATE_ESCAPED = 1, BH_FATE_CAPTURED = -1 };
-enum { BH_ORDER_ROWS = 1, BH_ORDER_IMPACT = 2 };     /* tile order of a persistent launch's queues */
+enum { BH_ORDER_ROWS = 1, BH_ORDER_IMPACT = 2, BH_ORDER_CRITICAL = 3 };   /* tile order of a persistent launch's queues */
 
 /* SceneGPU (scene.hpp:12-69): the scalars on the vacuum/radio path. The thin disk, sky and colour
    fields are out of scope (visible-mode presentation) and absent. */
@@ -211,12 +211,15 @@ typedef struct bh_batch_cfg {
     int use_graph;      /* capture and replay the launch as a hipGraph */
     int waves;          /* 0: kernel compiled for the compiler's register budget; 3 or 4: compiled for at
                            least that many resident waves per SIMD (fewer registers) */
-    int order;          /* tiles of each frame queued in BH_ORDER_ROWS (row) order or BH_ORDER_IMPACT
-                           order: by the impact parameter of the tile's centre ray, smallest first
-                           (the rays that orbit longest start first); 0 -> the default */
+    int order;          /* tiles of each frame queued in BH_ORDER_ROWS (row) order, BH_ORDER_IMPACT
+                           order (by the impact parameter of the tile's centre ray, smallest first) or
+                           BH_ORDER_CRITICAL order (by the predicted length of the tile's centre ray,
+                           longest first: -ln |min R(r) / r^4| over the spherical photon orbits' radii,
+                           which grows with the number of orbits near the critical curve); 0 -> the
+                           default */
     int prio_age;       /* > 0: a wave holding a ray that has taken prio_age steps raises its issue
                            priority; < 0: off; 0 -> the default. A NULL cfg (and no autotuned one)
-                           means, for the mixed tracer, 256 / 4 / 32 / 32 / graph / 0 / impact order /
+                           means, for the mixed tracer, 256 / 4 / 32 / 32 / graph / 0 / critical order /
                            256, for the FP64 tracer 256 / 8 / 64 / 48 / graph / 3 / row order / 256. */
 } bh_batch_cfg;
 
@@ -226,8 +229,9 @@ int bh_trace_batch(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_tr
                    const bh_batch_cfg* cfg, float* px_out, bh_stats* stats);
 
 /* Times bh_trace_batch over a grid of configurations on the given workload and returns the fastest
-   (best) with its kernel time (best_ms). tried (nullable, 8 doubles per configuration: block,
-   grid_per_cu, interval, refill_min, ms, waves, order, prio_age) receives up to max_tried rows;
+   (best) with its kernel time (best_ms): the faster of two timed launches after a warm-up one. tried
+   (nullable, 8 doubles per configuration: block, grid_per_cu, interval, refill_min, ms, waves, order,
+   prio_age) receives up to max_tried rows;
    *n_tried their number. */
 int bh_autotune(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_trace_opts* opts, bh_batch_cfg* best,
                 double* best_ms, double* tried, int max_tried, int* n_tried);

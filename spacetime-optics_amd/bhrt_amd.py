@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libbhrt.so")
 BH_OK, BH_ERR_ARG, BH_ERR_HIP, BH_ERR_NODEVICE, BH_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 MODE_VISIBLE, MODE_RADIO = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-ORDER_ROWS, ORDER_IMPACT = 1, 2
+ORDER_ROWS, ORDER_IMPACT, ORDER_CRITICAL = 1, 2, 3
 TURB_GRID, TURB_SHEARED = 0, 1
 EMIS_EXACT, EMIS_MLP = 0, 1
 

@@ -3,8 +3,8 @@
    Workgroups stay resident and pull rays from per-frame work queues (one atomic counter per frame;
    queue entries are the rays of 8 x 8 pixel tiles, so the rays a wave takes at once are image
    neighbours). The tiles of a frame are queued in row order or in a given order (the launcher sorts
-   them by the impact parameter of their centre ray, so the rays that orbit the hole longest start
-   first). Every `interval` integration steps a wave retires finished lanes (one ballot),
+   them by their centre ray's impact parameter, or by its predicted length, longest first, so the
+   rays that orbit the hole longest start first). Every `interval` integration steps a wave retires finished lanes (one ballot),
    writes their pixels, and refills the free lanes with new rays when at least `refill_min` lanes are
    free: lane 0 claims a contiguous range with one atomic add and each free lane takes the entry at
    its rank among the free lanes. A wave refills only from its current frame and moves to the next
@@ -20,6 +20,9 @@
 #include "trace_mixed.h"
 
 namespace bhp {
+
+constexpr int MAX_BLOCK = 256;   // largest workgroup of the persistent kernel
+constexpr int NSTAT = 5;         // per-ray statistics summed per lane (steps, accepted, flow, escaped, FP64)
 
 using bhd::RayResult;
 using bhd::TraceParams;
@@ -159,8 +162,13 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
     bool active = false;
     unsigned int ent = 0;   // the lane's queue entry (the work mapping recovers pixel and sample)
     typename T::State R;
-    double flux = 0.0;
-    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+    // per-lane sums over the lane's finished rays, in the lane's own LDS slots (updated once per ray,
+    // so they need no registers during the integration)
+    __shared__ double s_flux[MAX_BLOCK];
+    __shared__ unsigned long long s_stat[NSTAT][MAX_BLOCK];
+    const int tid = threadIdx.x;
+    s_flux[tid] = 0.0;
+    for (int i = 0; i < NSTAT; ++i) s_stat[i][tid] = 0ull;
     for (;;) {
         // retire-and-refill
         const unsigned long long freem = __ballot(!active);
@@ -216,24 +224,20 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
                     RayResult r;
                     const int f64 = T::result(R, r);
                     Wk.finish(cf, ent, r);
-                    flux += r.I[0];
-                    c0 += (unsigned long long)r.steps;
-                    c1 += (unsigned long long)r.accepted;
-                    c2 += (unsigned long long)r.flow_steps;
-                    c3 += r.fate == 0 ? 1ull : 0ull;
-                    c4 += (unsigned long long)f64;
+                    s_flux[tid] += r.I[0];
+                    s_stat[0][tid] += (unsigned long long)r.steps;
+                    s_stat[1][tid] += (unsigned long long)r.accepted;
+                    s_stat[2][tid] += (unsigned long long)r.flow_steps;
+                    s_stat[3][tid] += r.fate == 0 ? 1ull : 0ull;
+                    s_stat[4][tid] += (unsigned long long)f64;
                     active = false;
                 }
             }
         }
     }
     if (prio_age > 0) __builtin_amdgcn_s_setprio(0);
-    *flux_out = flux;
-    cnt_out[0] = c0;
-    cnt_out[1] = c1;
-    cnt_out[2] = c2;
-    cnt_out[3] = c3;
-    cnt_out[4] = c4;
+    *flux_out = s_flux[tid];
+    for (int i = 0; i < NSTAT; ++i) cnt_out[i] = s_stat[i][tid];
 }
 
 }  // namespace bhp

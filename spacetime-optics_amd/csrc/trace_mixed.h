@@ -211,7 +211,7 @@ struct EmisParams {
     float ln_nuc_per_B;   // ln(e / (2 pi m_e c))
     float ln_jconst;   // ln(sqrt(2) pi e^2 / (3 c))
     float ln_bnu_const;   // ln(2 h / c^2)
-    float h_over_mec2;    // h / (m_e c^2)  (x = h nu / (k T_e) = h nu / (m_e c^2 Theta))
+    float ln_h_over_mec2; // ln(h / (m_e c^2))  (x = h nu / (k T_e) = h nu / (m_e c^2 Theta))
     float ln_nu_obs;
     float ln_rg;
     float k2_inv_dl;   // 1 / (K2 table spacing)
@@ -227,22 +227,24 @@ __device__ __forceinline__ float table_lerp_f32(const float* __restrict__ t, int
     return t0 + (t1 - t0) * f;
 }
 
-// Thermal synchrotron: ln j and ln alpha from ln n_e, ln Theta, ln nu_f, ln nu_c, ln sin(theta_B)
-// (Leung et al. 2011 with the K2 table, Kirchhoff alpha = j / B_nu).
+// Thermal synchrotron: ln j, ln alpha and ln B_nu from ln n_e, ln Theta, ln nu_f, ln nu_c,
+// ln sin(theta_B) (Leung et al. 2011 with the K2 table, Kirchhoff alpha = j / B_nu). With
+// y = X^(1/3): sqrt(X) + 2^(11/12) X^(1/6) = X^(1/6) (y + 2^(11/12)), so ln j needs one exponential
+// and one logarithm; X is capped at e^240 (where j = exp(-y) is already 0 in FP32) so y stays finite.
 __device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const EmisParams& E, float ln_ne,
                                                   float ln_th, float ln_nu, float ln_nuc, float ln_sinb,
-                                                  float* ln_j, float* ln_a) {
+                                                  float* ln_j, float* ln_a, float* ln_bnu) {
     const float ln_nus = -1.5040774f + ln_nuc + 2.f * ln_th + ln_sinb;   // ln(2/9) + ...
-    const float lnX = ln_nu - ln_nus;
-    const float X = expf_(lnX), x3 = expf_(lnX * (1.f / 3.f));
-    const float brk = sqrtf_(X) + 1.887748625363387f * sqrtf_(x3);
+    const float lnX = fminf(ln_nu - ln_nus, 240.f);
+    const float y = expf_(lnX * (1.f / 3.f));
     const float lnk2 = table_lerp_f32(P.k2, P.k2_n, P.k2_l0, E.k2_inv_dl, ln_th);
-    *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + 2.f * lnf(brk) - x3;
-    // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x))
-    const float x = E.h_over_mec2 * expf_(ln_nu - ln_th);
-    const float ln_em1 = x < 1e-3f ? lnf(x) + 0.5f * x : lnf(expf_(x) - 1.f);
-    const float ln_bnu = E.ln_bnu_const + 3.f * ln_nu - ln_em1;
-    *ln_a = *ln_j - ln_bnu;
+    *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + lnX * (1.f / 3.f) + 2.f * lnf(y + 1.887748625363387f) - y;
+    // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x)); ln x is known
+    const float ln_x = E.ln_h_over_mec2 + ln_nu - ln_th;
+    const float x = expf_(ln_x);
+    const float ln_em1 = x < 1e-3f ? ln_x + 0.5f * x : (x > 30.f ? x : lnf(expf_(x) - 1.f));
+    *ln_bnu = E.ln_bnu_const + 3.f * ln_nu - ln_em1;
+    *ln_a = *ln_j - *ln_bnu;
 }
 
 // Turbulence density factor in FP32: the grid, cell-centred trilinear weights, edge clamps and
@@ -307,7 +309,7 @@ __device__ __forceinline__ float sheared_factor_f32(const TraceParams& P, float 
 
 // Geometry + plasma of one flow sample; returns false when nothing is emitted or absorbed.
 struct FlowSample {
-    float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl, ln_bnu;
+    float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl;
 };
 
 template <int RADIO>
@@ -322,15 +324,15 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     const float lnvert = -0.5f * zh * zh;
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
     const float lr = lnf(r);
-    float ln_ne = E.ln_n0 - E.n_index * lr + lnvert;
+    float dens = 1.f;   // turbulence factor x horizon softening: one logarithm for both
     if (P.turb_on) {
-        if constexpr (RADIO == bhd::RADIO_SHEARED) ln_ne += lnf(sheared_factor_f32(P, r, R, z, phid, td));
-        else ln_ne += lnf(turb_factor_f32(P, r, u, phid, td));
+        if constexpr (RADIO == bhd::RADIO_SHEARED) dens = sheared_factor_f32(P, r, R, z, phid, td);
+        else dens = turb_factor_f32(P, r, u, phid, td);
     }
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
     if (!(soft > 0.f)) return false;
-    ln_ne += lnf(soft);
+    const float ln_ne = E.ln_n0 - E.n_index * lr + lnvert + lnf(dens * soft);
     if (ln_ne < -69.07755f) return false;   // n_e < 1e-30
     const float ln_th = E.ln_theta0 - E.theta_index * lr;
     const float ln_B = 0.5f * (E.ln_bconst + ln_ne - lr);
@@ -350,8 +352,7 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     const float ut = __builtin_amdgcn_rsqf(norm);
     const float kdotu = ut * (1.f - Om * lam);
     if (!(kdotu > 0.f)) return false;
-    const float ln_kdotu = lnf(kdotu);
-    const float ln_nu = E.ln_nu_obs + lnf((float)Ecamd) + ln_kdotu;
+    const float ln_nu = E.ln_nu_obs + lnf((float)Ecamd * kdotu);
     const float u_ph = ut * (g_tph + g_phph * Om);
     const float kb = lam - kdotu * u_ph;
     const float bb = g_phph + u_ph * u_ph;
@@ -363,17 +364,18 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     f.ln_nuc = E.ln_nuc_per_B + ln_B;
     f.ln_sinb = 0.5f * lnf(sinb2);
     f.ln_g = E.ln_nu_obs - ln_nu;
-    f.ln_dl = ln_kdotu + lnf(Sigma) + lnf((float)hd) + E.ln_rg;
+    f.ln_dl = lnf(kdotu * Sigma * (float)hd) + E.ln_rg;   // k.u Sigma h: within FP32 range inside the flow
     return true;
 }
 
 // dI = g^3 B_nu (1 - e^-tau), dtau = alpha dl (trace.cuh:131-143 with emission.cuh:167-174)
-__device__ __forceinline__ void transfer(const EmisParams& E, const FlowSample& f, float ln_j, float ln_a,
-                                         double* dI, double* dtau) {
+__device__ __forceinline__ void transfer(const EmisParams& E, const FlowSample& f, float ln_a, float ln_bnu,
+                                         double* dI, double* dtau, float* etau) {
     const float tau = expf_(ln_a + f.ln_dl);
-    const float om = tau < 1e-4f ? tau * (1.f - 0.5f * tau) : 1.f - expf_(-tau);
-    const float ln_bnu = ln_j - ln_a;
+    const float e = expf_(-tau);   // also the step's transmission factor
+    const float om = tau < 1e-4f ? tau * (1.f - 0.5f * tau) : 1.f - e;
     *dtau = (double)tau;
+    *etau = e;
     *dI = (double)(expf_(3.f * f.ln_g + ln_bnu) * om);
 }
 
@@ -426,7 +428,7 @@ struct RayStateM {
     RayConst c;
     State s;
     double h, acc0, acc1, acc2, acc3, trans;
-    float psi, ppx, ppy, ppz;
+    float psi;
     int ncross, accepted, flow_steps, it, fate, fp64_attempts;
     bool done, k1_f32;
 };
@@ -450,14 +452,6 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
     R.trans = 1.0;
     R.psi = 0.f;
     R.ncross = R.accepted = R.flow_steps = R.it = R.fate = R.fp64_attempts = 0;
-    R.ppx = R.ppy = R.ppz = 0.f;
-    if (RADIO) {
-        float sp = sqrtf_(fmaxf(1.f - (float)(R.s.y2 * R.s.y2), 0.f));
-        float ph = (float)R.s.y4;
-        R.ppx = sp * __cosf(ph);
-        R.ppy = sp * __sinf(ph);
-        R.ppz = (float)R.s.y2;
-    }
     R.done = P.max_steps <= 0;
 }
 
@@ -496,31 +490,36 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     }
     ++R.accepted;
     if (RADIO) {
-        float sn = sqrtf_(fmaxf(1.f - (float)(n.y2 * n.y2), 0.f));
-        float ph = (float)n.y4;
-        float pnx = sn * __cosf(ph), pny = sn * __sinf(ph), pnz = (float)n.y2;
-        float cx = R.ppy * pnz - R.ppz * pny, cy = R.ppz * pnx - R.ppx * pnz, cz = R.ppx * pny - R.ppy * pnx;
-        R.psi += atan2_pos(sqrtf_(cx * cx + cy * cy + cz * cz), R.ppx * pnx + R.ppy * pny + R.ppz * pnz);
-        R.ppx = pnx;
-        R.ppy = pny;
-        R.ppz = pnz;
+        // angle between the radial directions of the previous and the new accepted state, in the
+        // frame rotated to the previous azimuth: p = (s0, 0, u0), q = (s1 cos dphi, s1 sin dphi, u1),
+        // with dphi taken in FP64 (no stored direction, no large-angle FP32 azimuth)
+        const float u0 = (float)R.s.y2, u1 = (float)n.y2;
+        const float s0 = sqrtf_(fmaxf((float)(1.0 - R.s.y2 * R.s.y2), 0.f));
+        const float s1 = sqrtf_(fmaxf((float)(1.0 - n.y2 * n.y2), 0.f));
+        float sd, cd;
+        __sincosf((float)(n.y4 - R.s.y4), &sd, &cd);
+        const float a1 = u0 * s1;
+        const float cx = a1 * sd, cy = a1 * cd - s0 * u1, cz = s0 * s1 * sd;
+        R.psi += atan2_pos(sqrtf_(cx * cx + cy * cy + cz * cz), s0 * s1 * cd + u0 * u1);
     }
     if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
     if (flow && n.y0 < M.r_cut) {
         ++R.flow_steps;
         FlowSample f;
         if (flow_sample<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
-            float ln_j, ln_a, fj, fa;
+            float ln_j, ln_a, ln_bnu, fj, fa;
             if (M.use_surrogate &&
                 surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
                                  f.ln_sinb, &fj, &fa)) {
                 ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
                 ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
+                ln_bnu = ln_j - ln_a;   // both within the table's floors: no cancellation
             } else {
-                synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a);
+                synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a, &ln_bnu);
             }
             double dI, dtau;
-            transfer(M.E, f, ln_j, ln_a, &dI, &dtau);
+            float etau;
+            transfer(M.E, f, ln_a, ln_bnu, &dI, &dtau, &etau);
             double cI = R.trans * dI;
             int ord = (int)(R.psi * 0.31830988618379067f);
             R.acc0 += cI;
@@ -528,7 +527,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             R.acc1 += ord == 1 ? cI : 0.0;
             R.acc2 += ord == 2 ? cI : 0.0;
             R.acc3 += ord >= 3 ? cI : 0.0;
-            R.trans *= (double)expf_(-(float)dtau);
+            R.trans *= (double)etau;
             if (R.trans < 1e-5) {
                 R.fate = -1;
                 R.done = true;

@@ -64,7 +64,8 @@ def test_batch_single_frame_every_config(c2, prec):
     for block, gpc, interval, refill, graph, order, age in [
             (256, 2, 8, 16, 1, bh.ORDER_ROWS, -1), (256, 2, 8, 16, 0, bh.ORDER_IMPACT, -1),
             (64, 1, 1, 1, 0, bh.ORDER_ROWS, 64), (128, 4, 32, 48, 1, bh.ORDER_IMPACT, 128),
-            (256, 8, 3, 64, 1, bh.ORDER_IMPACT, 1), (256, 4, 32, 32, 1, 0, 0)]:
+            (256, 8, 3, 64, 1, bh.ORDER_IMPACT, 1), (256, 4, 32, 32, 1, 0, 0),
+            (256, 4, 32, 32, 1, bh.ORDER_CRITICAL, 256), (128, 3, 16, 32, 0, bh.ORDER_CRITICAL, -1)]:
         got, st = t.trace_batch([(cfg["S"], cfg["F"])], o, block=block, grid_per_cu=gpc, interval=interval,
                                 refill_min=refill, use_graph=graph, order=order, prio_age=age)
         assert_same(got, want, f"prec {prec} cfg {(block, gpc, interval, refill, graph, order, age)}")
@@ -85,7 +86,7 @@ def test_batch_mixed_sizes_and_spins(c2, prec):
     o = t.opts(precision=prec)
     want, flux, cnt = render_each(t, frames, o)
     t.update_scene(cfg["S"])
-    for order, age in [(0, 0), (bh.ORDER_ROWS, -1), (bh.ORDER_IMPACT, 32)]:
+    for order, age in [(0, 0), (bh.ORDER_ROWS, -1), (bh.ORDER_IMPACT, 32), (bh.ORDER_CRITICAL, 256)]:
         got, st = t.trace_batch(frames, o, order=order, prio_age=age)
         assert_same(got, want, f"batch prec {prec} order {order} prio_age {age}")
         assert np.array_equal(stats_vec(st), cnt)
@@ -162,7 +163,7 @@ def test_batch_errors(c2):
     with pytest.raises(bh.BhError, match="MLP emissivity"):
         t.trace_batch([(cfg["S"], cfg["F"])], t.opts(precision=bh.PREC_FP64, emissivity=bh.EMIS_MLP))
     with pytest.raises(bh.BhError, match="order must be"):
-        t.trace_batch([(cfg["S"], cfg["F"])], order=3)
+        t.trace_batch([(cfg["S"], cfg["F"])], order=4)
     with pytest.raises(bh.BhError, match="prio_age must be"):
         t.trace_batch([(cfg["S"], cfg["F"])], prio_age=(1 << 24) + 1)
 
