@@ -7,13 +7,16 @@ grid already resident in HBM; the frame stays in device memory (bh_render_reside
 JSON line. The CPU baseline is the oracle's glibc build (the reference integrator's arithmetic,
 restated in C++) on a bounded random pixel sample of the same frame, over all host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes fp64,mixed,...] [--autotune]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--modes fp64,mixed,...] [--no-autotune]
+                    [--batch-frames N]
 
 Modes: fp64 / mixed / mixed_mlp run one launch per frame (bh_render_resident); p_* run the same
 tracers through the persistent batched kernel (bh_trace_batch, graph replay); p_mixed_f32far is
 the mixed tracer with FP32 stages outside the flow, p_mixed_fullflow the mixed tracer with emission,
 absorption and the transfer step cap over the whole flow (default: inside the emission region). `value` is the
 fastest mode whose frame is within 0.1% flux and 1% median per-pixel error of the FP64 frame.
+`animation_batch` reports the same mode over N frames of an animation (camera azimuth stepped per
+frame) in one persistent launch, beside the one-frame-per-step headline.
 """
 import argparse
 import json
@@ -169,7 +172,7 @@ FLUX_GATE, PIXEL_GATE = 1e-3, 1e-2   # a mode counts only below 0.1% flux and 1%
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-pixels", type=int, default=1048576)
@@ -178,10 +181,12 @@ def main():
     ap.add_argument("--no-mlp-train", action="store_true", help="skip MLP modes")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a device copy before the modes (profiling)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-sampling render")
+    ap.add_argument("--batch-frames", type=int, default=4,
+                    help="frames of the animation batch traced in one persistent launch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (rays shard with no "
-              f"exchange; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print(f"bench.py: the tracer runs on one GPU; --gpus {args.gpus} is not supported (rays shard with no "
+              f"exchange: one process per GPU would take frames or tile ranges from a shared counter)", file=sys.stderr)
         sys.exit(2)
     if bh.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
@@ -281,6 +286,32 @@ def main():
             log(f"bench: adaptive render ({name}): {st.ms_kernel:.1f} ms, {st.samples / 1e6:.2f} M samples "
                 f"({st.samples / st.ms_kernel * 1e-3:.1f} Msamples/s), {int((ns > 8).sum())} pixels refined")
 
+    # the batching path: an animation of several frames of the same workload (camera azimuth stepped
+    # by 0.05 rad per frame) in one persistent launch, with the head mode's launch configuration; the
+    # longest rays of one frame overlap the bulk of the next, so the per-frame time drops below the
+    # single-frame one (reported beside the headline, which stays one frame per step)
+    batch = None
+    if args.batch_frames > 0 and MODES[head_mode][2]:
+        prec_h, emis_h, _, f32_h, full_h = MODES[head_mode]
+        o = bh.Tracer.opts(precision=prec_h, emissivity=emis_h, jitter=0, spp=1, far_fp32=f32_h, full_flow=full_h)
+        cfg = bh.BatchCfg(**setup[f"autotune_{head_mode}"]["best"]) if f"autotune_{head_mode}" in setup else None
+        frames = []
+        for k in range(args.batch_frames):
+            Fk = type(F).from_buffer_copy(F)
+            Fk.ph = F.ph + 0.05 * k
+            frames.append((S, Fk))
+        tracer.trace_batch(frames, o, cfg=cfg, resident=True)
+        kms = []
+        for _ in range(max(args.steps, 1)):
+            _, st = tracer.trace_batch(frames, o, cfg=cfg, resident=True)
+            kms.append(st.ms_kernel)
+        ms = float(np.mean(kms))
+        rays = args.batch_frames * F.W * F.H
+        batch = {"mode": head_mode, "frames": args.batch_frames, "kernel_ms": round(ms, 3),
+                 "ms_per_frame": round(ms / args.batch_frames, 3), "mrays_per_s": round(rays / ms * 1e-3, 2)}
+        log(f"bench: batch of {args.batch_frames} frames ({head_mode}): {ms:.2f} ms, "
+            f"{ms / args.batch_frames:.3f} ms per frame, {rays / ms * 1e-3:.1f} Mrays/s")
+
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_pixels} pixels")
@@ -313,6 +344,7 @@ def main():
         "modes": {m: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for m, r in results.items()},
         "setup": setup,
         "adaptive_render_spp8_128": adaptive or None,
+        "animation_batch": batch,
     }
     if fp64:
         line["speedup_vs_fp64"] = round(head["mrays_per_s"] / fp64["mrays_per_s"], 3)

@@ -229,7 +229,7 @@ int bh_trace_batch(bh_ctx* ctx, const bh_batch_frame* frames, int n, const bh_tr
                    const bh_batch_cfg* cfg, float* px_out, bh_stats* stats);
 
 /* Times bh_trace_batch over a grid of configurations on the given workload and returns the fastest
-   (best) with its kernel time (best_ms): the faster of two timed launches after a warm-up one. tried
+   (best) with its kernel time (best_ms): the median of three timed launches after a warm-up one. tried
    (nullable, 8 doubles per configuration: block, grid_per_cu, interval, refill_min, ms, waves, order,
    prio_age) receives up to max_tried rows;
    *n_tried their number. */

@@ -1490,12 +1490,13 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
         bh_stats s{};
         int rc = launch_batch(c, frames, n, opts, &g, nullptr, &s);   // first run: warm caches and graph
         if (rc) return rc;
-        double best_of = 1e300;   // the faster of two timed runs (launch-to-launch spread is ~10 %)
-        for (int rep = 0; rep < 2; ++rep) {
+        double t[3];   // the median of three timed runs (launch-to-launch spread is up to 10 %)
+        for (int rep = 0; rep < 3; ++rep) {
             rc = launch_batch(c, frames, n, opts, &g, nullptr, &s);
             if (rc) return rc;
-            best_of = std::min(best_of, (double)s.ms_kernel);
+            t[rep] = s.ms_kernel;
         }
+        const double best_of = std::max(std::min(t[0], t[1]), std::min(std::max(t[0], t[1]), t[2]));
         *ms = best_of;
         if (tried && count < max_tried) {
             double* row = tried + 8 * (size_t)count;

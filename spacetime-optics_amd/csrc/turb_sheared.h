@@ -14,7 +14,7 @@
    distance), each a (z/R, ln R, phi) grid, phi fastest. A flow sample reads two slices with
    trilinear filtering and interpolates between them in its own emission time (slow light).
 
-   This build's choices: the slices are float arrays in HBM (453 MB at the reference's
+   Choices here: the slices are float arrays in HBM (453 MB at the reference's
    512 x 192 x 48 x 24; MI355X has 288 GB) filtered in software — wrap in phi, clamp in ln R and
    z/R, texel centres at (i + 0.5) / n, weights rounded to 1/256 like the 8-bit fraction of a
    texture unit's filter — so the oracle can restate every value; double-precision sine, cosine, log
