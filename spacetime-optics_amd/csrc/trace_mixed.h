@@ -393,12 +393,17 @@ __device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T,
           g2 <= (float)(T.n2 - 1)))
         return false;
     const int i0 = min((int)g0, T.n0 - 2), i1 = min((int)g1, T.n1 - 2), i2 = min((int)g2, T.n2 - 2);
-    if (T.bad[((size_t)i2 * (T.n1 - 1) + i1) * (T.n0 - 1) + i0]) return false;
-    const float f0 = g0 - i0, f1 = g1 - i1, f2 = g2 - i2;
-    const size_t s1 = (size_t)T.n0, s2 = (size_t)T.n0 * T.n1;
-    const float2* b = (const float2*)T.tab + (size_t)i2 * s2 + (size_t)i1 * s1 + i0;
+    // the cell's trust flag and its eight corners are loaded together (the indices are in range
+    // whatever the flag says), so the lookup costs one memory round trip; 32-bit offsets (bh_mlp_bake
+    // keeps the table at most 2^24 nodes)
+    const unsigned int s1 = (unsigned int)T.n0, s2 = (unsigned int)(T.n0 * T.n1);
+    const unsigned char flag = T.bad[((unsigned int)i2 * (unsigned int)(T.n1 - 1) + (unsigned int)i1) *
+                                         (unsigned int)(T.n0 - 1) + (unsigned int)i0];
+    const float2* b = (const float2*)T.tab + ((unsigned int)i2 * s2 + (unsigned int)i1 * s1 + (unsigned int)i0);
     const float2 v000 = b[0], v100 = b[1], v010 = b[s1], v110 = b[s1 + 1];
     const float2 v001 = b[s2], v101 = b[s2 + 1], v011 = b[s2 + s1], v111 = b[s2 + s1 + 1];
+    if (flag) return false;
+    const float f0 = g0 - i0, f1 = g1 - i1, f2 = g2 - i2;
     float x00 = v000.x + (v100.x - v000.x) * f0, x10 = v010.x + (v110.x - v010.x) * f0;
     float x01 = v001.x + (v101.x - v001.x) * f0, x11 = v011.x + (v111.x - v011.x) * f0;
     float y00 = v000.y + (v100.y - v000.y) * f0, y10 = v010.y + (v110.y - v010.y) * f0;
