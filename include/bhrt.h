@@ -323,6 +323,41 @@ typedef struct bh_tone_params {     /* ToneParams (renderer.hpp:48-59), radio fi
    order tint, vignette, grain and dithered 8-bit rounding (k_tonemap, render.cu:360-407). */
 int bh_tonemap(bh_ctx* ctx, const float* px, int W, int H, const bh_tone_params* params, unsigned char* rgb8);
 
+/* HUD elements (HudGlyph, HudRect, renderer.hpp:61-62): a glyph of the font at (x, y) (top left of
+   its cell, pixels) scaled by `scale`, colour rgba = R | G << 8 | B << 16 | A << 24; a rectangle
+   [x0, x1) x [y0, y1) blended with its colour's alpha, drawn in increasing `layer`. */
+typedef struct bh_hud_glyph {
+    float x, y, scale;
+    int glyph;
+    uint32_t rgba;
+} bh_hud_glyph;
+typedef struct bh_hud_rect {
+    int x0, y0, x1, y1;
+    uint32_t rgba;
+    int layer;
+} bh_hud_rect;
+
+/* A bitmap font: `glyphs` cells of cell_w x cell_h 8-bit coverage values, glyph-major then row-major
+   (the reference compiles one in: FONT_ATLAS, common/font_atlas.h). advance (pixels per character
+   at scale 1) and baseline are for the host-side text layout. */
+typedef struct bh_font {
+    const unsigned char* atlas;
+    int cell_w, cell_h, glyphs;
+    float advance;
+    int baseline;
+} bh_font;
+
+/* Renderer::tonemap with its HUD (render.cu:811-846): the radio tone map of bh_tonemap, then the
+   rectangles stable-sorted by layer, then every glyph's shadow (offset max(1.5, H / 720) px, black,
+   3/4 of the glyph's alpha), then every glyph, bilinearly sampled from the font (k_hud_rects,
+   k_hud_glyphs, render.cu:409-446). At most 1024 rectangles and 4096 glyphs are drawn, as the
+   reference does. Each element is blended in list order (the reference's blocks blend overlapping
+   elements in whatever order they run). Errors: font NULL with glyphs, a glyph index outside the
+   font, a scale outside (0, 64]. */
+int bh_tonemap_hud(bh_ctx* ctx, const float* px, int W, int H, const bh_tone_params* params,
+                   const bh_hud_glyph* glyphs, int n_glyphs, const bh_hud_rect* rects, int n_rects,
+                   const bh_font* font, unsigned char* rgb8);
+
 /* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context
    needed; device 0). fn: 0 exp, 1 log, 2 expm1, 3 cbrt, 4 x^-0.2, 5 sin, 6 cos, 7 atan2(x, y),
    8 acos, 9 sqrt, 10 x / y. Used by the tests that pin the device functions bit for bit. */

@@ -166,6 +166,10 @@ void or_frame_stats(const float* px, int W, int H, float hide_direct, double per
                     double* lum_max);
 void or_tonemap_radio(const float* px, int W, int H, int colormap, float exposure, float gamma, float overlay,
                       float hide_direct, float fade, float vignette, float grain, uint32_t seed, uint8_t* rgb);
+struct OrGlyph;
+struct OrRect;
+void or_hud_draw(uint8_t* rgb, int W, int H, const struct OrGlyph* glyphs, int ng, const struct OrRect* rects, int nr,
+                 const uint8_t* atlas, int cell_w, int cell_h, float off);
 
 #ifdef __cplusplus
 }

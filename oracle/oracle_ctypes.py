@@ -136,6 +136,8 @@ class Oracle:
         L.or_frame_stats.argtypes = [_F, C.c_int, C.c_int, C.c_float, C.c_double, _D, _D]
         L.or_tonemap_radio.argtypes = ([_F, C.c_int, C.c_int, C.c_int] + [C.c_float] * 7 +
                                        [C.c_uint32, C.POINTER(C.c_uint8)])
+        L.or_hud_draw.argtypes = [C.POINTER(C.c_uint8), C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                  C.POINTER(C.c_uint8), C.c_int, C.c_int, C.c_float]
         L.or_escape_angles.argtypes = [_D, _D, _D, _D]
         L.or_trace_ray.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.c_double, C.c_double, C.POINTER(OrRay)]
         L.or_trace_pixels.restype = C.c_int
@@ -320,6 +322,20 @@ class Oracle:
         self.lib.or_tonemap_radio(px.ctypes.data_as(_F), W, H, colormap, exposure, gamma, overlay, hide_direct, fade,
                                   vignette, grain, seed, rgb.ctypes.data_as(C.POINTER(C.c_uint8)))
         return rgb
+
+    def hud_draw(self, rgb, glyphs, rects, atlas, off):
+        """HUD over an H x W x 3 uint8 image (copied): glyphs / rects as sequences of (x, y, scale, glyph,
+        rgba) / (x0, y0, x1, y1, rgba, layer); atlas glyphs x cell_h x cell_w uint8."""
+        out = np.ascontiguousarray(rgb, dtype=np.uint8).copy()
+        H, W = out.shape[:2]
+        g = np.array([tuple(v) for v in glyphs], dtype=[("x", "<f4"), ("y", "<f4"), ("s", "<f4"), ("g", "<i4"), ("c", "<u4")])
+        r = np.array([tuple(v) for v in rects], dtype=[("x0", "<i4"), ("y0", "<i4"), ("x1", "<i4"), ("y1", "<i4"),
+                                                        ("c", "<u4"), ("l", "<i4")])
+        at = np.ascontiguousarray(atlas, dtype=np.uint8)
+        self.lib.or_hud_draw(out.ctypes.data_as(C.POINTER(C.c_uint8)), W, H, g.ctypes.data_as(C.c_void_p), len(g),
+                             r.ctypes.data_as(C.c_void_p), len(r), at.ctypes.data_as(C.POINTER(C.c_uint8)),
+                             at.shape[2], at.shape[1], C.c_float(off))
+        return out
 
     def flow_step(self, S, r, u, phi, t, lam, Ecam, h):
         dI = C.c_double()

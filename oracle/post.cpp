@@ -11,7 +11,10 @@
    - or_frame_stats: the displayed-intensity statistics of Renderer::render (render.cu:771-791).
    - or_tonemap_radio: k_tonemap's radio branch with colormap, burn, order tint, vignette
      (smoothstep, common.hpp:47-50), grain and dithered rounding (render.cu:334-407); the display
-     gamma through bh_math.h's FP64 exp/log (the reference calls powf). */
+     gamma through bh_math.h's FP64 exp/log (the reference calls powf).
+   - or_hud_draw: the HUD rectangles and glyphs blended onto the 8-bit image (render.cu:409-446,
+     811-846), every element in list order (the reference draws the elements of one kernel launch
+     concurrently, so where two overlap its result depends on scheduling). */
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -241,6 +244,77 @@ void or_tonemap_radio(const float* px, int W, int H, int cmap, float exposure, f
                 rgb[3 * (size_t)p + k] = (uint8_t)std::fmin(std::fmax(q * 255.f + 0.5f + d[k], 0.f), 255.f);
             }
         }
+}
+
+
+// HUD on an 8-bit RGB image (Renderer::tonemap, render.cu:811-846, with k_hud_rects and
+// k_hud_glyphs, render.cu:409-446): the rectangles stable-sorted by layer, then every glyph's shadow
+// (offset off px, black, 3/4 of the glyph's alpha), then every glyph, each blended in list order.
+// glyphs: ng records {float x, y, scale; int glyph; uint32 rgba}; rects: nr records {int x0, y0, x1,
+// y1; uint32 rgba; int layer}; atlas: glyphs x ch x cw bytes.
+struct OrGlyph {
+    float x, y, scale;
+    int glyph;
+    uint32_t rgba;
+};
+struct OrRect {
+    int x0, y0, x1, y1;
+    uint32_t rgba;
+    int layer;
+};
+
+void or_hud_draw(uint8_t* rgb, int W, int H, const OrGlyph* glyphs, int ng, const OrRect* rects_in, int nr,
+                 const uint8_t* atlas, int cw_cell, int ch_cell, float off) {
+    std::vector<OrRect> rects(rects_in, rects_in + nr);
+    std::stable_sort(rects.begin(), rects.end(), [](const OrRect& a, const OrRect& b) { return a.layer < b.layer; });
+    for (const OrRect& R : rects) {
+        const float a = (float)((R.rgba >> 24) & 255) / 255.f;
+        const int s[3] = {(int)(R.rgba & 255), (int)((R.rgba >> 8) & 255), (int)((R.rgba >> 16) & 255)};
+        for (int Y = std::max(R.y0, 0); Y < std::min(R.y1, H); ++Y)
+            for (int X = std::max(R.x0, 0); X < std::min(R.x1, W); ++X) {
+                uint8_t* o = rgb + 3 * ((size_t)Y * W + X);
+                for (int k = 0; k < 3; ++k) {
+                    const float d = (float)o[k];
+                    o[k] = (uint8_t)std::fmin(d * (1.f - a) + (float)s[k] * a + 0.5f, 255.f);
+                }
+            }
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool shadow = pass == 0;
+        const float dx = shadow ? off : 0.f, dy = shadow ? off : 0.f;
+        for (int gi = 0; gi < ng; ++gi) {
+            const OrGlyph& G = glyphs[gi];
+            const int cw = (int)std::ceil((float)cw_cell * G.scale) + 1, ch = (int)std::ceil((float)ch_cell * G.scale) + 1;
+            float c[3] = {(float)(G.rgba & 255) / 255.f, (float)((G.rgba >> 8) & 255) / 255.f,
+                          (float)((G.rgba >> 16) & 255) / 255.f};
+            float am = (float)((G.rgba >> 24) & 255) / 255.f;
+            if (shadow) {
+                c[0] = c[1] = c[2] = 0.f;
+                am *= 0.75f;
+            }
+            const int ox = (int)std::floor(G.x + dx), oy = (int)std::floor(G.y + dy);
+            auto A = [&](int xx, int yy) -> float {
+                if (xx < 0 || yy < 0 || xx >= cw_cell || yy >= ch_cell) return 0.f;
+                return (float)atlas[((size_t)G.glyph * ch_cell + yy) * cw_cell + xx] / 255.f;
+            };
+            for (int iy = 0; iy < ch; ++iy)
+                for (int ix = 0; ix < cw; ++ix) {
+                    const int X = ox + ix, Y = oy + iy;
+                    if (X < 0 || Y < 0 || X >= W || Y >= H) continue;
+                    const float u = ((float)ix + 0.5f) / G.scale - 0.5f, v = ((float)iy + 0.5f) / G.scale - 0.5f;
+                    const int u0 = (int)std::floor(u), v0 = (int)std::floor(v);
+                    const float fu = u - (float)u0, fv = v - (float)v0;
+                    const float a = ((1.f - fv) * ((1.f - fu) * A(u0, v0) + fu * A(u0 + 1, v0)) +
+                                     fv * ((1.f - fu) * A(u0, v0 + 1) + fu * A(u0 + 1, v0 + 1))) * am;
+                    if (a <= 0.f) continue;
+                    uint8_t* o = rgb + 3 * ((size_t)Y * W + X);
+                    for (int k = 0; k < 3; ++k) {
+                        const float d = (float)o[k];
+                        o[k] = (uint8_t)std::fmin(d * (1.f - a) + c[k] * 255.f * a + 0.5f, 255.f);
+                    }
+                }
+        }
+    }
 }
 
 }  // extern "C"

@@ -26,6 +26,7 @@ EXPORTS = [
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
     "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap", "bh_sheared_slices",
+    "bh_tonemap_hud",
 ]
 MLP_NPARAM = 8706
 SHOT_VALUES = 20   # values per frame of bhh_build_shot
@@ -93,6 +94,28 @@ class BatchCfg(C.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HudGlyph(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("scale", C.c_float), ("glyph", C.c_int), ("rgba", C.c_uint32)]
+
+
+class HudRect(C.Structure):
+    _fields_ = [("x0", C.c_int), ("y0", C.c_int), ("x1", C.c_int), ("y1", C.c_int), ("rgba", C.c_uint32),
+                ("layer", C.c_int)]
+
+
+class Font(C.Structure):
+    _fields_ = [("atlas", C.POINTER(C.c_uint8)), ("cell_w", C.c_int), ("cell_h", C.c_int), ("glyphs", C.c_int),
+                ("advance", C.c_float), ("baseline", C.c_int)]
+
+
+def hud_arrays(glyphs, rects):
+    """ctypes arrays of HudGlyph / HudRect from sequences of tuples (x, y, scale, glyph, rgba) and
+    (x0, y0, x1, y1, rgba, layer)."""
+    ga = (HudGlyph * max(len(glyphs), 1))(*[HudGlyph(*g) for g in glyphs])
+    ra = (HudRect * max(len(rects), 1))(*[HudRect(*r) for r in rects])
+    return ga, ra
 
 
 class PostParams(C.Structure):
@@ -194,6 +217,9 @@ def load():
     L.bh_post_radio.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(PostParams), C.c_double, _D, _D]
     L.bh_tonemap.restype = C.c_int
     L.bh_tonemap.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(ToneParams), C.POINTER(C.c_uint8)]
+    L.bh_tonemap_hud.restype = C.c_int
+    L.bh_tonemap_hud.argtypes = [C.c_void_p, _F, C.c_int, C.c_int, C.POINTER(ToneParams), C.POINTER(HudGlyph), C.c_int,
+                                 C.POINTER(HudRect), C.c_int, C.POINTER(Font), C.POINTER(C.c_uint8)]
     L.bh_math_eval.restype = C.c_int
     L.bh_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
     L.bh_destroy.argtypes = [C.c_void_p]
@@ -510,14 +536,27 @@ class Tracer:
         return out, lp.value, lm.value
 
     def tonemap(self, img, colormap=0, exposure=1.0, gamma=1.0, overlay=0.0, hide_direct=0.0, fade=1.0, vignette=0.0,
-                grain=0.0, seed=0):
-        """Radio tone map of an H x W x 4 float32 frame to H x W x 3 bytes."""
+                grain=0.0, seed=0, glyphs=None, rects=None, atlas=None, advance=0.0, baseline=0):
+        """Radio tone map of an H x W x 4 float32 frame to H x W x 3 bytes; with glyphs / rects (sequences
+        of (x, y, scale, glyph, rgba) / (x0, y0, x1, y1, rgba, layer)) and atlas (glyphs x cell_h x
+        cell_w uint8), the HUD is drawn on top (bh_tonemap_hud)."""
         px = np.ascontiguousarray(img, dtype=np.float32)
         H, W = px.shape[:2]
         rgb = np.zeros((H, W, 3), dtype=np.uint8)
         tp = ToneParams(colormap, exposure, gamma, overlay, hide_direct, fade, vignette, grain, seed)
-        self._check(self.L.bh_tonemap(self.ctx, px.ctypes.data_as(_F), W, H, C.byref(tp),
-                                      rgb.ctypes.data_as(C.POINTER(C.c_uint8))), "bh_tonemap")
+        if glyphs is None and rects is None:
+            self._check(self.L.bh_tonemap(self.ctx, px.ctypes.data_as(_F), W, H, C.byref(tp),
+                                          rgb.ctypes.data_as(C.POINTER(C.c_uint8))), "bh_tonemap")
+            return rgb
+        glyphs, rects = list(glyphs or []), list(rects or [])
+        ga, ra = hud_arrays(glyphs, rects)
+        font = None
+        if atlas is not None:
+            at = np.ascontiguousarray(atlas, dtype=np.uint8)
+            font = Font(at.ctypes.data_as(C.POINTER(C.c_uint8)), at.shape[2], at.shape[1], at.shape[0], advance, baseline)
+        self._check(self.L.bh_tonemap_hud(self.ctx, px.ctypes.data_as(_F), W, H, C.byref(tp), ga, len(glyphs), ra,
+                                          len(rects), C.byref(font) if font is not None else None,
+                                          rgb.ctypes.data_as(C.POINTER(C.c_uint8))), "bh_tonemap_hud")
         return rgb
 
     # --- persistent batches -------------------------------------------------------------------

@@ -1606,7 +1606,7 @@ void bh_destroy(bh_ctx* c) {
                     c->d_fates,  c->d_mlp,    c->d_adam_m, c->d_adam_v,  c->d_stab,   c->d_sbad,   c->d_bframes,
                     c->d_queue,  c->d_bout,   c->d_samp1,  c->d_acc,     c->d_ystat,  c->d_extra,  c->d_chunks,
                     c->d_offsets, c->d_bsum,  c->d_nsamp,  c->d_fpart,   c->d_samp2,  c->d_order,  c->d_ohist,
-                    c->d_post_lev, c->d_post_tmp, c->d_post_wt, c->d_post_rgb, c->d_ts, c->d_tsv};
+                    c->d_post_lev, c->d_post_tmp, c->d_post_wt, c->d_post_rgb, c->d_hud, c->d_ts, c->d_tsv};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

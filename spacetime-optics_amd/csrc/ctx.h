@@ -81,6 +81,8 @@ struct bh_ctx {
     size_t post_wt_cap = 0;
     void* d_post_rgb = nullptr;
     size_t post_rgb_cap = 0;
+    void* d_hud = nullptr;                // HUD rectangles, glyphs and font atlas of bh_tonemap_hud
+    size_t hud_cap = 0;
     float4* d_bout = nullptr;
     size_t bout_cap = 0;
     hipGraphExec_t graph = nullptr;

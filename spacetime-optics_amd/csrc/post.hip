@@ -196,6 +196,66 @@ int grow(bh_ctx* c, void** p, size_t* cap, size_t bytes) {
 
 inline dim3 grid1(long long n) { return dim3((unsigned int)((n + POST_BLOCK - 1) / POST_BLOCK)); }
 
+// HUD over the tone-mapped bytes, one work-item per pixel: every rectangle (sorted by layer on the
+// host), then every glyph's shadow, then every glyph, in list order - the blends of k_hud_rects and
+// k_hud_glyphs (render.cu:409-446) in the same float operations. The element loops are wave-uniform,
+// so each element's fields arrive through scalar loads; a glyph touches only the pixels of its cell.
+__global__ __launch_bounds__(POST_BLOCK) void k_hud(unsigned char* __restrict__ rgb, int W, int H,
+                                                    const bh_hud_rect* __restrict__ rects, int nr,
+                                                    const bh_hud_glyph* __restrict__ glyphs, int ng,
+                                                    const unsigned char* __restrict__ atlas, int cell_w, int cell_h,
+                                                    float off) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)W * H) return;
+    const int X = (int)(p % W), Y = (int)(p / W);
+    unsigned char* o = rgb + 3 * p;
+    float d0 = (float)o[0], d1 = (float)o[1], d2 = (float)o[2];
+    bool touched = false;
+    for (int i = 0; i < nr; ++i) {
+        const bh_hud_rect R = rects[i];
+        if (X < R.x0 || X >= R.x1 || Y < R.y0 || Y >= R.y1) continue;
+        const float a = (float)((R.rgba >> 24) & 255u) / 255.f;
+        d0 = (float)(unsigned char)fminf(d0 * (1.f - a) + (float)(int)(R.rgba & 255u) * a + 0.5f, 255.f);
+        d1 = (float)(unsigned char)fminf(d1 * (1.f - a) + (float)(int)((R.rgba >> 8) & 255u) * a + 0.5f, 255.f);
+        d2 = (float)(unsigned char)fminf(d2 * (1.f - a) + (float)(int)((R.rgba >> 16) & 255u) * a + 0.5f, 255.f);
+        touched = true;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        const float dxy = pass == 0 ? off : 0.f;
+        for (int i = 0; i < ng; ++i) {
+            const bh_hud_glyph G = glyphs[i];
+            const int cw = (int)ceilf((float)cell_w * G.scale) + 1, ch = (int)ceilf((float)cell_h * G.scale) + 1;
+            const int ix = X - (int)floorf(G.x + dxy), iy = Y - (int)floorf(G.y + dxy);
+            if (ix < 0 || iy < 0 || ix >= cw || iy >= ch) continue;
+            float r = (float)(G.rgba & 255u) / 255.f, g = (float)((G.rgba >> 8) & 255u) / 255.f,
+                  b = (float)((G.rgba >> 16) & 255u) / 255.f, am = (float)((G.rgba >> 24) & 255u) / 255.f;
+            if (pass == 0) {
+                r = g = b = 0.f;
+                am *= 0.75f;
+            }
+            const float u = ((float)ix + 0.5f) / G.scale - 0.5f, v = ((float)iy + 0.5f) / G.scale - 0.5f;
+            const int u0 = (int)floorf(u), v0 = (int)floorf(v);
+            const float fu = u - (float)u0, fv = v - (float)v0;
+            const unsigned char* cell = atlas + (size_t)G.glyph * cell_h * cell_w;
+            auto A = [&](int xx, int yy) -> float {
+                return (xx < 0 || yy < 0 || xx >= cell_w || yy >= cell_h) ? 0.f : (float)cell[yy * cell_w + xx] / 255.f;
+            };
+            const float a = ((1.f - fv) * ((1.f - fu) * A(u0, v0) + fu * A(u0 + 1, v0)) +
+                             fv * ((1.f - fu) * A(u0, v0 + 1) + fu * A(u0 + 1, v0 + 1))) * am;
+            if (a <= 0.f) continue;
+            d0 = (float)(unsigned char)fminf(d0 * (1.f - a) + r * 255.f * a + 0.5f, 255.f);
+            d1 = (float)(unsigned char)fminf(d1 * (1.f - a) + g * 255.f * a + 0.5f, 255.f);
+            d2 = (float)(unsigned char)fminf(d2 * (1.f - a) + b * 255.f * a + 0.5f, 255.f);
+            touched = true;
+        }
+    }
+    if (touched) {
+        o[0] = (unsigned char)d0;
+        o[1] = (unsigned char)d1;
+        o[2] = (unsigned char)d2;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -287,8 +347,14 @@ int bh_post_radio(bh_ctx* c, float* px, int W, int H, const bh_post_params* para
     return BH_OK;
 }
 
-int bh_tonemap(bh_ctx* c, const float* px, int W, int H, const bh_tone_params* params, unsigned char* rgb8) {
-    if (!c || !px || !rgb8 || !params || W <= 0 || H <= 0) return BH_ERR_ARG;
+static int copy_rgb(bh_ctx* c, int W, int H, unsigned char* rgb8) {
+    PCK(c, hipMemcpyAsync(rgb8, c->d_post_rgb, 3 * (size_t)W * H, hipMemcpyDeviceToHost, c->st));
+    PCK(c, hipStreamSynchronize(c->st));
+    return BH_OK;
+}
+
+// the radio tone map into the context's device byte buffer
+static int tone_device(bh_ctx* c, const float* px, int W, int H, const bh_tone_params* params) {
     if ((long long)W * H > (1ll << 28)) {
         c->err = "frame too large for the tone map (2^28 pixels)";
         return BH_ERR_ARG;
@@ -307,9 +373,57 @@ int bh_tonemap(bh_ctx* c, const float* px, int W, int H, const bh_tone_params* p
     hipLaunchKernelGGL(k_tone_radio, grid1((long long)npix), dim3(POST_BLOCK), 0, c->st,
                        (const float4*)c->d_post_tmp, W, H, *params, (unsigned char*)c->d_post_rgb);
     PCK(c, hipGetLastError());
-    PCK(c, hipMemcpyAsync(rgb8, c->d_post_rgb, 3 * npix, hipMemcpyDeviceToHost, c->st));
-    PCK(c, hipStreamSynchronize(c->st));
     return BH_OK;
+}
+
+int bh_tonemap(bh_ctx* c, const float* px, int W, int H, const bh_tone_params* params, unsigned char* rgb8) {
+    if (!c || !px || !rgb8 || !params || W <= 0 || H <= 0) return BH_ERR_ARG;
+    const int rc = tone_device(c, px, W, H, params);
+    return rc ? rc : copy_rgb(c, W, H, rgb8);
+}
+
+int bh_tonemap_hud(bh_ctx* c, const float* px, int W, int H, const bh_tone_params* params, const bh_hud_glyph* glyphs,
+                   int n_glyphs, const bh_hud_rect* rects, int n_rects, const bh_font* font, unsigned char* rgb8) {
+    if (!c || !px || !rgb8 || !params || W <= 0 || H <= 0 || n_glyphs < 0 || n_rects < 0) return BH_ERR_ARG;
+    if ((n_glyphs > 0 && !glyphs) || (n_rects > 0 && !rects)) return BH_ERR_ARG;
+    const int ng = std::min(n_glyphs, 4096), nr = std::min(n_rects, 1024);   // the reference's limits
+    if (ng > 0) {
+        if (!font || !font->atlas || font->cell_w < 1 || font->cell_h < 1 || font->glyphs < 1 ||
+            font->cell_w > 256 || font->cell_h > 256) {
+            c->err = "HUD glyphs need a font (atlas, cell_w and cell_h in [1, 256], glyphs >= 1)";
+            return BH_ERR_ARG;
+        }
+        for (int i = 0; i < ng; ++i) {
+            if (glyphs[i].glyph < 0 || glyphs[i].glyph >= font->glyphs) {
+                c->err = "HUD glyph index outside the font";
+                return BH_ERR_ARG;
+            }
+            if (!(glyphs[i].scale > 0.f && glyphs[i].scale <= 64.f) || !std::isfinite(glyphs[i].x) ||
+                !std::isfinite(glyphs[i].y)) {
+                c->err = "HUD glyph scale must be in (0, 64] and its position finite";
+                return BH_ERR_ARG;
+            }
+        }
+    }
+    int rc = tone_device(c, px, W, H, params);
+    if (rc) return rc;
+    if (ng == 0 && nr == 0) return copy_rgb(c, W, H, rgb8);
+    std::vector<bh_hud_rect> sorted(rects, rects + nr);
+    std::stable_sort(sorted.begin(), sorted.end(), [](const bh_hud_rect& a, const bh_hud_rect& b) { return a.layer < b.layer; });
+    const size_t rbytes = sizeof(bh_hud_rect) * (size_t)nr, gbytes = sizeof(bh_hud_glyph) * (size_t)ng;
+    const size_t abytes = ng > 0 ? (size_t)font->glyphs * font->cell_h * font->cell_w : 0;
+    const size_t goff = (rbytes + 255) & ~(size_t)255, aoff = goff + ((gbytes + 255) & ~(size_t)255);
+    if ((rc = grow(c, &c->d_hud, &c->hud_cap, aoff + abytes + 1))) return rc;
+    unsigned char* base = (unsigned char*)c->d_hud;
+    if (nr) PCK(c, hipMemcpyAsync(base, sorted.data(), rbytes, hipMemcpyHostToDevice, c->st));
+    if (ng) PCK(c, hipMemcpyAsync(base + goff, glyphs, gbytes, hipMemcpyHostToDevice, c->st));
+    if (ng) PCK(c, hipMemcpyAsync(base + aoff, font->atlas, abytes, hipMemcpyHostToDevice, c->st));
+    const float off = std::max(1.5f, (float)H / 720.f);
+    hipLaunchKernelGGL(k_hud, grid1((long long)W * H), dim3(POST_BLOCK), 0, c->st, (unsigned char*)c->d_post_rgb, W, H,
+                       (const bh_hud_rect*)base, nr, (const bh_hud_glyph*)(base + goff), ng, base + aoff,
+                       ng ? font->cell_w : 1, ng ? font->cell_h : 1, off);
+    PCK(c, hipGetLastError());
+    return copy_rgb(c, W, H, rgb8);
 }
 
 }  // extern "C"

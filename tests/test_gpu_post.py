@@ -1,4 +1,4 @@
-"""Radio post step and tone map on the GPU (bh_post_radio, bh_tonemap) against the oracle's
+"""Radio post step, tone map and HUD on the GPU (bh_post_radio, bh_tonemap, bh_tonemap_hud) against the oracle's
 restatement (oracle/post.cpp): the same float operations in the same order, so the blurred frames
 are bit-identical, the statistics equal and the 8-bit images byte-identical. Inputs: a traced
 256^2 radio frame and seeded random frames of ragged sizes; every pyramid depth of the beam blur
@@ -66,6 +66,52 @@ def test_tonemap_matches_oracle(radio, orc, opts):
         assert got.shape == want.shape
         nbad = int((got != want).any(axis=-1).sum())
         assert nbad == 0, f"{name} {opts}: {nbad} pixels differ"
+
+
+def hud_case(rng, W, H, ng, nr, glyphs_in_font):
+    glyphs = [(float(rng.uniform(-30, W)), float(rng.uniform(-30, H)), float(rng.uniform(0.2, 3.0)),
+               int(rng.integers(0, glyphs_in_font)), int(rng.integers(0, 2 ** 32))) for _ in range(ng)]
+    rects = [(int(rng.integers(-20, W)), int(rng.integers(-20, H)), int(rng.integers(-20, W + 20)),
+              int(rng.integers(-20, H + 20)), int(rng.integers(0, 2 ** 32)), int(rng.integers(0, 3))) for _ in range(nr)]
+    return glyphs, rects
+
+
+def test_hud_matches_oracle(radio, orc):
+    """bh_tonemap_hud against the oracle's tone map + HUD (render.cu:409-446, 811-846): random
+    overlapping rectangles in three layers and glyphs of random scale and position (partly off the
+    image) from a synthetic 99-glyph atlas, byte for byte; plus a title-like line of glyphs."""
+    t, img = radio
+    rng = np.random.default_rng(9)
+    atlas = rng.integers(0, 256, (99, 44, 24), dtype=np.uint8)
+    atlas[:, :4] = 0
+    for name, a in frames(img):
+        H, W = a.shape[:2]
+        glyphs, rects = hud_case(rng, W, H, 300, 40, 99)
+        glyphs += [(10.0 + 21.6 * 0.6 * i, 20.0, 0.6, (7 * i) % 99, 0xF0FFFFFF) for i in range(20)]
+        opts = dict(exposure=2.0, vignette=0.1, grain=0.01, seed=17)
+        got = t.tonemap(a, glyphs=glyphs, rects=rects, atlas=atlas, advance=21.6, baseline=34, **opts)
+        want = orc.hud_draw(orc.tonemap_radio(a, **opts), glyphs, rects, atlas, max(1.5, H / 720.0))
+        nbad = int((got != want).any(axis=-1).sum())
+        assert nbad == 0, f"{name}: {nbad} pixels differ"
+        plain = t.tonemap(a, **opts)
+        assert (got != plain).any()
+    # rectangles only (no font needed) and the reference's element limits (1024 rects, 4096 glyphs)
+    a = frames(img)[0][1]
+    _, rects = hud_case(rng, 256, 256, 0, 1100, 1)
+    got = t.tonemap(a, rects=rects)
+    want = orc.hud_draw(orc.tonemap_radio(a), [], rects[:1024], np.zeros((1, 1, 1), np.uint8), 1.5)
+    assert np.array_equal(got, want)
+
+
+def test_hud_errors(radio):
+    t, img = radio
+    atlas = np.zeros((10, 8, 6), np.uint8)
+    with pytest.raises(bh.BhError, match="outside the font"):
+        t.tonemap(img, glyphs=[(1.0, 1.0, 1.0, 10, 0xFFFFFFFF)], atlas=atlas)
+    with pytest.raises(bh.BhError, match="scale must be"):
+        t.tonemap(img, glyphs=[(1.0, 1.0, 0.0, 1, 0xFFFFFFFF)], atlas=atlas)
+    with pytest.raises(bh.BhError, match="need a font"):
+        t.tonemap(img, glyphs=[(1.0, 1.0, 1.0, 1, 0xFFFFFFFF)])
 
 
 def test_post_errors(radio):

@@ -170,3 +170,45 @@ def test_smooth_exposure_closed_forms():
     want0 = 1.0 / np.exp((np.log(1.0) + w * np.log(4.0)) / (1 + w))
     assert abs(bh.smooth_exposure(lp, 1.0, 1.0)[0] - want0) < 1e-13
     assert np.all(bh.smooth_exposure(np.zeros(6), 0.9, 2.0) == 1.0)
+
+
+def test_hud_rect_closed_forms(orc):
+    """Rectangles (render.cu:434-446): alpha 255 paints the colour, alpha 0 leaves the image, partial
+    alpha is d (1 - a) + s a rounded; clipped to the image; drawn by layer, list order within one."""
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (30, 40, 3), dtype=np.uint8)
+    out = orc.hud_draw(img, [], [(5, 6, 15, 12, 0xFF204080, 0)], np.zeros((1, 4, 4), np.uint8), 1.5)
+    assert np.all(out[6:12, 5:15] == [0x80, 0x40, 0x20]) and np.array_equal(out[:6], img[:6])
+    out = orc.hud_draw(img, [], [(0, 0, 40, 30, 0x00FFFFFF, 0)], np.zeros((1, 4, 4), np.uint8), 1.5)
+    assert np.array_equal(out, img)
+    a = np.float32(0x55) / np.float32(255)
+    out = orc.hud_draw(img, [], [(-10, -10, 100, 100, 0x55000000, 0)], np.zeros((1, 4, 4), np.uint8), 1.5)
+    want = np.floor(img.astype(np.float32) * (np.float32(1) - a) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(out, want)
+    # layer 1 drawn after layer 0 whatever the list order
+    two = [(0, 0, 10, 10, 0xFF0000FF, 1), (0, 0, 10, 10, 0xFF00FF00, 0)]
+    out = orc.hud_draw(img, [], two, np.zeros((1, 4, 4), np.uint8), 1.5)
+    assert np.all(out[:10, :10] == [255, 0, 0])
+
+
+def test_hud_glyph_closed_forms(orc):
+    """Glyphs (render.cu:409-432): at scale 1 and integer position the bilinear sample is the atlas
+    coverage itself; white at full alpha gives d (1 - a) + 255 a; the shadow is the same coverage,
+    black, 3/4 alpha, offset by `off`; a blank cell draws nothing."""
+    cw, ch = 6, 9
+    atlas = np.zeros((3, ch, cw), np.uint8)
+    atlas[1] = np.random.default_rng(4).integers(0, 256, (ch, cw))
+    img = np.full((40, 40, 3), 100, np.uint8)
+    out = orc.hud_draw(img, [(10.0, 12.0, 1.0, 1, 0xFFFFFFFF)], [], atlas, 20.0)   # shadow far away
+    a = atlas[1].astype(np.float32) / np.float32(255)
+    want = np.floor(np.float32(100) * (1 - a) + np.float32(255) * a + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(out[12:12 + ch, 10:10 + cw, 0], want)
+    am = np.float32(0.75)
+    ws = np.floor(np.float32(100) * (1 - a * am) + np.float32(0.5)).astype(np.uint8)
+    sh = out[32:32 + ch, 30:30 + cw, 0][:8, :10]   # the shadow's cell clipped by the image
+    assert np.array_equal(sh, ws[:sh.shape[0], :sh.shape[1]])
+    assert np.array_equal(orc.hud_draw(img, [(3.0, 4.0, 2.5, 0, 0xFFFFFFFF)], [], atlas, 1.5), img)
+    # scale 2: the cell covers ceil(2 cw) + 1 pixels and samples between atlas texels
+    big = orc.hud_draw(img, [(5.0, 5.0, 2.0, 1, 0xFFFFFFFF)], [], atlas, 40.0)
+    changed = np.argwhere((big != img).any(axis=-1))
+    assert changed[:, 0].max() < 5 + 2 * ch + 1 and changed[:, 1].max() < 5 + 2 * cw + 1
