@@ -63,6 +63,17 @@ int bhh_write_png(const char* path, int w, int h, int channels, const unsigned c
    and lp[j] > 0 (sigma = 0: the frame alone); out[i] = 1 where no frame contributes. */
 void bhh_smooth_exposure(const double* lp, int n, double key, double sigma, double* out);
 
+/* HUD layout of frame `frame` of a shot (make_hud, main.cpp:195-300) in a W x H image for a font of
+   cell height cell_h and advance `advance` px: up to glyph_cap glyphs and rect_cap rectangles, the
+   numbers in *n_glyphs / *n_rects (may exceed the caps). Returns 0, or -1 with the error in err. */
+int bhh_make_hud(const char* ini, const char* sets, int frame, int W, int H, int cell_h, float advance,
+                 bh_hud_glyph* glyphs, int glyph_cap, int* n_glyphs, bh_hud_rect* rects, int rect_cap, int* n_rects,
+                 char* err, size_t errlen);
+
+/* Writes a font file for the HUD ("BHFONT1\n", int32 cell_w, cell_h, glyphs, baseline, float32
+   advance, then the glyphs x cell_h x cell_w coverage bytes). Returns 0 or -1. */
+int bhh_save_font(const char* path, const bh_font* font);
+
 /* Self-test rows (8 x 6: a, incl_deg, traced -, traced +, analytic -, analytic +) on `device`. */
 int bhh_selftest(int device, double* rows, char* err, size_t errlen);
 

@@ -26,7 +26,7 @@ EXPORTS = [
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
     "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap", "bh_sheared_slices",
-    "bh_tonemap_hud",
+    "bh_tonemap_hud", "bhh_make_hud", "bhh_save_font",
 ]
 MLP_NPARAM = 8706
 SHOT_VALUES = 20   # values per frame of bhh_build_shot
@@ -245,6 +245,12 @@ def load():
     L.bhh_write_png.restype = C.c_int
     L.bhh_write_png.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int]
     L.bhh_smooth_exposure.argtypes = [_D, C.c_int, C.c_double, C.c_double, _D]
+    L.bhh_make_hud.restype = C.c_int
+    L.bhh_make_hud.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                               C.POINTER(HudGlyph), C.c_int, C.POINTER(C.c_int), C.POINTER(HudRect), C.c_int,
+                               C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
+    L.bhh_save_font.restype = C.c_int
+    L.bhh_save_font.argtypes = [C.c_char_p, C.POINTER(Font)]
     L.bhh_selftest.restype = C.c_int
     L.bhh_selftest.argtypes = [C.c_int, _D, C.c_char_p, C.c_size_t]
     _lib = L
@@ -305,6 +311,50 @@ def write_png(path, img, level=4):
     if load().bhh_write_png(str(path).encode(), a.shape[1], a.shape[0], ch, a.ctypes.data_as(C.POINTER(C.c_uint8)),
                             level) != 0:
         raise BhError(f"cannot write {path}")
+
+
+def make_hud(ini, sets="", frame=0, W=1920, H=1080, cell_h=44, advance=21.6):
+    """HUD layout of one frame of a shot (bhh_make_hud): (glyphs, rects) as lists of (x, y, scale,
+    glyph, rgba) and (x0, y0, x1, y1, rgba, layer)."""
+    L = load()
+    ng, nr = C.c_int(), C.c_int()
+    err = C.create_string_buffer(512)
+    cap = 4096
+    ga, ra = (HudGlyph * cap)(), (HudRect * 1024)()
+    if L.bhh_make_hud(str(ini).encode(), sets.encode(), frame, W, H, cell_h, advance, ga, cap, C.byref(ng), ra, 1024,
+                      C.byref(nr), err, 512) != 0:
+        raise BhError(f"bhh_make_hud: {err.value.decode()}")
+    glyphs = [(g.x, g.y, g.scale, g.glyph, g.rgba) for g in ga[:min(ng.value, cap)]]
+    rects = [(r.x0, r.y0, r.x1, r.y1, r.rgba, r.layer) for r in ra[:min(nr.value, 1024)]]
+    return glyphs, rects
+
+
+def save_font(path, atlas, advance=None, baseline=0):
+    """Writes a HUD font file from a glyphs x cell_h x cell_w uint8 coverage atlas (glyph g = ASCII
+    32 + g for g < 95, then U+00B5, U+00B0, U+00D7, U+00B1); advance defaults to 0.9 cell widths."""
+    at = np.ascontiguousarray(atlas, dtype=np.uint8)
+    n, ch, cw = at.shape
+    adv = 0.9 * cw if advance is None else advance
+    f = Font(at.ctypes.data_as(C.POINTER(C.c_uint8)), cw, ch, n, adv, baseline)
+    if load().bhh_save_font(str(path).encode(), C.byref(f)) != 0:
+        raise BhError(f"cannot write {path}")
+
+
+def font_from_header(path):
+    """(atlas, baseline) from a C header in the layout of bhrt's common/font_atlas.h: FONT_CELL_W,
+    FONT_CELL_H, FONT_GLYPHS, FONT_BASELINE constants and the FONT_ATLAS byte list. For converting a
+    bhrt install's own font (bhrt lays its text out with 21.6 px per character at scale 1)."""
+    import re
+    text = open(path).read()
+    dim = {k: int(re.search(rf"{k}\s*=\s*(\d+)", text).group(1))
+           for k in ("FONT_CELL_W", "FONT_CELL_H", "FONT_GLYPHS", "FONT_BASELINE")}
+    body = text[text.index("FONT_ATLAS"):]
+    body = body[body.index("{") + 1:body.index("}")]
+    vals = np.array([int(v) for v in re.findall(r"\d+", body)], dtype=np.uint8)
+    n = dim["FONT_GLYPHS"] * dim["FONT_CELL_H"] * dim["FONT_CELL_W"]
+    if vals.size != n:
+        raise BhError(f"{path}: {vals.size} atlas bytes, expected {n}")
+    return vals.reshape(dim["FONT_GLYPHS"], dim["FONT_CELL_H"], dim["FONT_CELL_W"]), dim["FONT_BASELINE"]
 
 
 def smooth_exposure(lp, key, sigma):

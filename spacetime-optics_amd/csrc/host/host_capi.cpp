@@ -5,6 +5,7 @@
 
 #include "../../../include/bhrt_host.h"
 #include "config.h"
+#include "hud.h"
 #include "image_io.h"
 #include "renderer.h"
 #include "scene.h"
@@ -218,6 +219,51 @@ void bhh_smooth_exposure(const double* lp, int n, double key, double sigma, doub
     if (!lp || !out || n <= 0) return;
     const std::vector<double> e = bh::smooth_exposure(std::vector<double>(lp, lp + n), key, sigma);
     for (int i = 0; i < n; ++i) out[i] = e[(size_t)i];
+}
+
+int bhh_make_hud(const char* ini, const char* sets, int frame, int W, int H, int cell_h, float advance,
+                 bh_hud_glyph* glyphs, int glyph_cap, int* n_glyphs, bh_hud_rect* rects, int rect_cap, int* n_rects,
+                 char* err, size_t errlen) {
+    bh::Config c;
+    std::string e;
+    if (W <= 0 || H <= 0 || cell_h <= 0 || !(advance > 0.f)) {
+        copy_err("bad image size or font metrics", err, errlen);
+        return -1;
+    }
+    bh_scene S;
+    bh::SceneTables T;
+    bh::SceneInfo si;
+    bh::TracerOptions opt;
+    bh::Shot sh;
+    if (!load(ini, sets, &c, &e) || !bh::build_scene(c, &S, &T, &si, &opt, &e, false) || !bh::build_shot(c, &sh, &e)) {
+        copy_err(e, err, errlen);
+        return -1;
+    }
+    if (frame < 0 || frame >= (int)sh.frames.size()) {
+        copy_err("frame index outside the shot", err, errlen);
+        return -1;
+    }
+    const bh::ShotFrame& f = sh.frames[(size_t)frame];
+    const bh_frame F = bh::make_frame(sh, f, S.a, W, H, 1, 0);
+    const bh::HudLayout L = bh::make_hud(c, sh, f, F, si, S, W, H, cell_h, advance);
+    if (n_glyphs) *n_glyphs = (int)L.glyphs.size();
+    if (n_rects) *n_rects = (int)L.rects.size();
+    for (int i = 0; i < (int)L.glyphs.size() && i < glyph_cap && glyphs; ++i) glyphs[i] = L.glyphs[(size_t)i];
+    for (int i = 0; i < (int)L.rects.size() && i < rect_cap && rects; ++i) rects[i] = L.rects[(size_t)i];
+    return 0;
+}
+
+int bhh_save_font(const char* path, const bh_font* font) {
+    if (!path || !font || !font->atlas || font->cell_w < 1 || font->cell_h < 1 || font->glyphs < 1) return -1;
+    bh::Font f;
+    f.cell_w = font->cell_w;
+    f.cell_h = font->cell_h;
+    f.glyphs = font->glyphs;
+    f.baseline = font->baseline;
+    f.advance = font->advance;
+    f.atlas.assign(font->atlas, font->atlas + (size_t)font->cell_w * font->cell_h * font->glyphs);
+    std::string e;
+    return bh::save_font(path, f, &e) ? 0 : -1;
 }
 
 int bhh_selftest(int device, double* rows, char* err, size_t errlen) {

@@ -4,8 +4,9 @@
    sampling between render.spp and render.spp_max, then in radio mode the beam blur of post.blur x
    post.beam_uas and the frame's display statistics); the exposure of every frame follows from the
    smoothed statistics of all frames (post.exposure*, main.cpp:569-576), and each frame is tone
-   mapped and written as out/frames/frame_NNNNN.png (main.cpp:578-603). The HUD text overlay is not
-   drawn. With output.hdr = 1 the linear HDR frames (I, I_n1, I_n2, I_n>=3) are written as well:
+   mapped and written as out/frames/frame_NNNNN.png (main.cpp:578-603), with the HUD text overlay
+   (make_hud, main.cpp:195-300) when hud.enabled (default 1) and a font file is given (hud.font, or
+   --font FILE; the reference compiles its font in). With output.hdr = 1 the linear HDR frames (I, I_n1, I_n2, I_n>=3) are written as well:
    "BHF1" + int32 W, H, 4 + W*H*4 float32, top row first. A shot with a [batch] section (frames,
    spins, sizes, frame_dt_m) is traced in one persistent launch instead: frame i has spin
    spins[i % #spins], size sizes[(i / #spins) % #sizes] squared and time i * frame_dt_m. */
@@ -21,6 +22,7 @@
 
 #include "../../../include/bhrt_host.h"
 #include "config.h"
+#include "hud.h"
 #include "image_io.h"
 #include "renderer.h"
 #include "scene.h"
@@ -32,7 +34,8 @@ double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::
 
 void usage() {
     std::fprintf(stderr,
-                 "usage: bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B] [--list] [--selftest]\n");
+                 "usage: bhrt shot.ini [--out DIR] [--set section.key=value ...] [--frames A:B] [--font FILE] [--list] "
+                 "[--selftest]\n");
 }
 
 void mkdirs(const std::string& p) {
@@ -56,8 +59,16 @@ bool write_frame(const std::string& path, const bh::HdrFrame& f) {
 // statistics of all frames, radio tone map, PNG per frame (and the HDR frame with output.hdr = 1).
 // Visible-mode frames (vacuum geodesics: the disk and sky shading are not part of this tracer) are
 // written as HDR only.
+// What the HUD layout needs besides the configuration and the shot
+struct HudInputs {
+    const bh::Font* font = nullptr;   // null: no overlay
+    bh::SceneInfo si;
+    bh_scene S{};
+};
+
 bool tone_and_write(const bh::Config& cfg, const bh::Shot& shot, bh::Renderer& R, bool radio,
-                    const std::vector<int>& index, const std::vector<bh::HdrFrame>& frames, const std::string& out) {
+                    const std::vector<int>& index, const std::vector<bh::HdrFrame>& frames, const std::string& out,
+                    const HudInputs& hud) {
     const size_t n = frames.size();
     if (!radio) {
         for (size_t i = 0; i < n; ++i) {
@@ -91,7 +102,17 @@ bool tone_and_write(const bh::Config& cfg, const bh::Shot& shot, bh::Renderer& R
         tp.vignette = (float)cfg.num("post.vignette", 0.15);
         tp.grain = (float)cfg.num("post.grain", 0.006);
         tp.seed = 0x51ED27u + (uint32_t)idx * 7919u;
-        if (!R.tonemap(frames[i], tp, &rgb)) {
+        bool ok;
+        if (hud.font) {
+            const bh_frame F = bh::make_frame(shot, f, hud.S.a, frames[i].W, frames[i].H, 1, 0);
+            const bh::HudLayout L = bh::make_hud(cfg, shot, f, F, hud.si, hud.S, frames[i].W, frames[i].H,
+                                                 hud.font->cell_h, hud.font->advance);
+            const bh_font fv = hud.font->view();
+            ok = R.tonemap(frames[i], tp, L.glyphs, L.rects, &fv, &rgb);
+        } else {
+            ok = R.tonemap(frames[i], tp, &rgb);
+        }
+        if (!ok) {
             std::fprintf(stderr, "bhrt: tone map of frame %d: %s\n", idx, R.error().c_str());
             return false;
         }
@@ -145,9 +166,11 @@ int main(int argc, char** argv) {
     }
     bool list = false;
     int fa = 0, fb = -1;
+    std::string font_path;
     for (int i = 2; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--set" && i + 1 < argc) cfg.apply_set(argv[++i]);
+        else if (a == "--font" && i + 1 < argc) font_path = argv[++i];
         else if (a == "--out" && i + 1 < argc) cfg.set("output.dir", argv[++i]);
         else if (a == "--frames" && i + 1 < argc) std::sscanf(argv[++i], "%d:%d", &fa, &fb);
         else if (a == "--list") list = true;
@@ -294,7 +317,7 @@ int main(int argc, char** argv) {
                 return 1;
             }
         }
-        if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, imgs, out)) return 1;
+        if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, imgs, out, HudInputs{})) return 1;
         std::printf("done: batch of %d frames, %.3g rays in %.1f ms kernel / %.2f s wall (%.1f Mrays/s)\n", nb,
                     (double)st.samples, st.ms_kernel, tb, st.samples / std::max(st.ms_kernel, 1e-9) * 1e-3);
         return 0;
@@ -331,7 +354,23 @@ int main(int argc, char** argv) {
     }
     double tr = wall() - t_render;
     // phase 2: exposure, tone map, PNG
-    if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, done, out)) return 1;
+    HudInputs hud;
+    bh::Font font;
+    if (S.mode == BH_MODE_RADIO && cfg.flag("hud.enabled", true)) {
+        std::string fp = font_path.empty() ? cfg.str("hud.font", "") : font_path;
+        if (!fp.empty() && fp[0] != '/' && font_path.empty()) fp = cfg.dir() + "/" + fp;
+        if (fp.empty()) {
+            std::printf("  HUD: no font (hud.font or --font); frames are written without the overlay\n");
+        } else if (!bh::load_font(fp, &font, &err)) {
+            std::fprintf(stderr, "bhrt: %s\n", err.c_str());
+            return 1;
+        } else {
+            hud.font = &font;
+            hud.si = si;
+            hud.S = S;
+        }
+    }
+    if (!tone_and_write(cfg, shot, R, S.mode == BH_MODE_RADIO, index, done, out, hud)) return 1;
     FILE* m = std::fopen((out + "/meta.json").c_str(), "w");
     if (m) {
         std::fprintf(m, "{\"shot\": \"%s\", \"frames\": %d, \"fps\": %g, \"width\": %d, \"height\": %d}\n", shot.name.c_str(),

@@ -55,6 +55,14 @@ bool Renderer::tonemap(const HdrFrame& in, const bh_tone_params& tp, std::vector
     return bh_tonemap(ctx_, in.px.data(), in.W, in.H, &tp, rgb8->data()) == BH_OK;
 }
 
+bool Renderer::tonemap(const HdrFrame& in, const bh_tone_params& tp, const std::vector<bh_hud_glyph>& glyphs,
+                       const std::vector<bh_hud_rect>& rects, const bh_font* font, std::vector<uint8_t>* rgb8) {
+    if (!ctx_) return false;
+    rgb8->resize((size_t)in.W * in.H * 3);
+    return bh_tonemap_hud(ctx_, in.px.data(), in.W, in.H, &tp, glyphs.data(), (int)glyphs.size(), rects.data(),
+                          (int)rects.size(), font, rgb8->data()) == BH_OK;
+}
+
 std::vector<double> smooth_exposure(const std::vector<double>& lp, double key, double sigma) {
     const long n = (long)lp.size();
     std::vector<double> out((size_t)n, 1.0);

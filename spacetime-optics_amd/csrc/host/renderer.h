@@ -37,6 +37,9 @@ public:
                 HdrFrame* out, bh_stats* st);
     // Renderer::tonemap (render.cu:811-846) of a radio frame to 8-bit RGB, without the HUD overlay.
     bool tonemap(const HdrFrame& in, const bh_tone_params& tp, std::vector<uint8_t>* rgb8);
+    // ... with the HUD's rectangles and glyphs drawn in `font` on top (bh_tonemap_hud).
+    bool tonemap(const HdrFrame& in, const bh_tone_params& tp, const std::vector<bh_hud_glyph>& glyphs,
+                 const std::vector<bh_hud_rect>& rects, const bh_font* font, std::vector<uint8_t>* rgb8);
     // One persistent launch over several frames (each with its own scene): pixel-centre or sample-0
     // rays, images one after the other in `out`.
     bool render_batch(const std::vector<bh_batch_frame>& frames, const bh_trace_opts& opts,

@@ -140,3 +140,39 @@ def test_cli_writes_tone_mapped_frames(tmp_path):
         assert img[..., 0].max() > 150   # exposure puts the 99.5th percentile near key 0.92
         assert img[0, 0].max() <= 4      # the corner stays dark (grain and dither only)
     assert os.path.exists(os.path.join(tmp_path, "meta.json"))
+
+
+def test_cli_draws_hud_with_font_file(tmp_path):
+    """bhrt --font FILE: the frames equal the ones without a font except inside the HUD's blocks
+    (the layout of bhh_make_hud for the same frame), where text and bars are drawn."""
+    import os
+    import subprocess
+
+    from helpers import ROOT, read_png
+    rng = np.random.default_rng(12)
+    atlas = np.zeros((99, 44, 24), dtype=np.uint8)
+    # coverage only where a font's letters are (rows between cap height and baseline, inner columns),
+    # as the layout's text boxes assume; a full random cell would spill past them
+    atlas[:, 6:30, 4:18] = rng.integers(0, 256, (99, 24, 14), dtype=np.uint8)
+    font = os.path.join(tmp_path, "font.bin")
+    bh.save_font(font, atlas, advance=21.6, baseline=34)
+    exe = os.path.join(ROOT, "spacetime-optics_amd", "bhrt")
+    sets = ["camera.width=160", "camera.height=160", "render.spp=1", "render.spp_max=1", "shot.frames=1",
+            "flow.turbulence=0", "hud.style=radio", "hud.title=M87*", "post.overlay=0.5"]
+    imgs = []
+    for extra in ([], ["--font", font]):
+        out = os.path.join(tmp_path, "f" if extra else "n")
+        cmd = [exe, shot("c2_radio_1024.ini"), "--out", out] + extra
+        for s_ in sets:
+            cmd += ["--set", s_]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        imgs.append(read_png(os.path.join(out, "frames", "frame_00000.png")))
+    plain, hud = imgs
+    changed = (plain != hud).any(axis=-1)
+    assert changed.sum() > 50
+    glyphs, rects = bh.make_hud(shot("c2_radio_1024.ini"), ";".join(sets), 0, 160, 160, 44, 21.6)
+    inside = np.zeros_like(changed)
+    for x0, y0, x1, y1, _, _ in rects:
+        inside[max(y0, 0):max(y1, 0), max(x0, 0):max(x1, 0)] = True
+    assert not (changed & ~inside).any()
