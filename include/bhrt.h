@@ -204,8 +204,9 @@ typedef struct bh_batch_frame {
 } bh_batch_frame;
 
 typedef struct bh_batch_cfg {
-    int block;          /* threads per workgroup: 64, 128 or 256 (0 -> 256) */
-    int grid_per_cu;    /* resident workgroups per compute unit (0 -> the default) */
+    int block;          /* threads per workgroup: 64, 128 or 256 (0 -> the default) */
+    int grid_per_cu;    /* resident workgroups per compute unit (0 -> the default); capped at what the
+                           CU holds at once for the kernel and block size */
     int interval;       /* integration steps between retire/refill (0 -> the default) */
     int refill_min;     /* free lanes needed to refill (0 -> the default) */
     int use_graph;      /* capture and replay the launch as a hipGraph */
@@ -219,7 +220,7 @@ typedef struct bh_batch_cfg {
                            default */
     int prio_age;       /* > 0: a wave holding a ray that has taken prio_age steps raises its issue
                            priority; < 0: off; 0 -> the default. A NULL cfg (and no autotuned one)
-                           means, for the mixed tracer, 256 / 4 / 32 / 32 / graph / 0 / critical order /
+                           means, for the mixed tracer, 128 / 4 / 32 / 32 / graph / 3 / critical order /
                            256, for the FP64 tracer 256 / 8 / 64 / 48 / graph / 3 / row order / 256. */
 } bh_batch_cfg;
 

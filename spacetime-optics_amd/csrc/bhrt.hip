@@ -893,23 +893,34 @@ PersistFn pick_persist(int prec, int radio, int waves) {
     }
 }
 
+// Grid of a persistent launch: grid_per_cu workgroups per CU, but no more than the CU holds at once
+// for this kernel and block size (register file, LDS). A workgroup beyond that waits for a slot and
+// only starts once others have drained most of the queue, so its rays become the frame's tail.
+int persist_grid(const bh_ctx* c, const void* k, int block, int grid_per_cu) {
+    int resident = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k, block, 0) == hipSuccess && resident > 0)
+        grid_per_cu = std::min(grid_per_cu, resident);
+    return c->n_cu * grid_per_cu;
+}
+
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
 
 // Defaults: the autotuner's choices on the 1024^2 230 GHz frame (MI355X). The mixed tracer's launch is
 // bound by its longest rays (over 10x the mean step count): queueing them first (critical order) and
-// raising the priority of waves that hold rays older than 256 steps cut its frame time by a quarter,
-// and with that the compiler's own register budget (2 waves per SIMD) matches a forced 3.
-// The FP64 tracer (at most 3x the mean) is throughput-bound: 3 waves per SIMD, more resident
-// workgroups and longer retire intervals, row order.
+// raising the priority of waves that hold rays older than 256 steps cut its frame time by a quarter;
+// four 128-lane workgroups per CU (2 waves per SIMD of the 3 its registers allow) keep each wave's
+// share of the SIMD, and so the latency of the longest rays, high.
+// The FP64 tracer (at most 3x the mean) is throughput-bound: 3 waves per SIMD (all the CU holds at
+// 164 registers), longer retire intervals, row order.
 bh_batch_cfg batch_defaults(int prec) {
     bh_batch_cfg d{};
     const bool mixed = prec == BH_PREC_MIXED;
-    d.block = 256;
+    d.block = mixed ? 128 : 256;
     d.grid_per_cu = mixed ? 4 : 8;
     d.interval = mixed ? 32 : 64;
     d.refill_min = mixed ? 32 : 48;
     d.use_graph = 1;
-    d.waves = mixed ? 0 : 3;
+    d.waves = 3;
     d.order = mixed ? BH_ORDER_CRITICAL : BH_ORDER_ROWS;
     d.prio_age = 256;
     return d;
@@ -1030,7 +1041,8 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
         }
     }
     const int prio = cfg.prio_age > 0 ? cfg.prio_age : 0;
-    const int grid = c->n_cu * cfg.grid_per_cu;
+    PersistFn k = pick_persist(o.precision, radio_kind(c, mode), cfg.waves);
+    const int grid = persist_grid(c, (const void*)k, cfg.block, cfg.grid_per_cu);
     rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof(bhp::BatchFrame) * (size_t)n);
     if (rc) return rc;
     rc = ensure(c, &c->d_queue, &c->queue_cap, (size_t)n);
@@ -1038,7 +1050,6 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid);
     if (rc) return rc;
     HIPCK(c, hipMemcpyAsync(c->d_bframes, hb.data(), sizeof(bhp::BatchFrame) * (size_t)n, hipMemcpyHostToDevice, c->st));
-    PersistFn k = pick_persist(o.precision, radio_kind(c, mode), cfg.waves);
     const bhp::BatchFrame* dframes = (const bhp::BatchFrame*)c->d_bframes;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, c->st, c->d_queue, n);
@@ -1129,7 +1140,11 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     if (rc) return rc;
     rc = prepare_turbulence(c, c->S, f);
     if (rc) return rc;
-    const int grid = c->n_cu * cfg.grid_per_cu;
+    const int rk = radio_kind(c, c->S.mode);
+    const auto k1 = pick_adaptive<bhp::SampleWork>(o.precision, rk, cfg.waves);
+    const auto k2 = pick_adaptive<bhp::ChunkWork>(o.precision, rk, cfg.waves);
+    const int grid = std::min(persist_grid(c, (const void*)k1, cfg.block, cfg.grid_per_cu),
+                              persist_grid(c, (const void*)k2, cfg.block, cfg.grid_per_cu));
     const int prio = cfg.prio_age > 0 ? cfg.prio_age : 0;
     const bool ordered = cfg.order != BH_ORDER_ROWS;
     bhp::BatchFrame B[2];
@@ -1169,13 +1184,12 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
         return rc;
     const bhp::BatchFrame* dB = (const bhp::BatchFrame*)c->d_bframes;
     const bool radio = c->S.mode == BH_MODE_RADIO;
-    const int rk = radio_kind(c, c->S.mode);
     HIPCK(c, hipEventRecord(c->ev0, c->st));
     HIPCK(c, hipMemcpyAsync(c->d_bframes, B, sizeof B, hipMemcpyHostToDevice, c->st));
     // pass 1
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
     if (ordered) enqueue_tile_order(c->st, dB + 1, 1, (unsigned int)tiles, c->d_ohist, cfg.order);
-    hipLaunchKernelGGL(pick_adaptive<bhp::SampleWork>(o.precision, rk, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
+    hipLaunchKernelGGL(k1, dim3(grid), dim3(cfg.block), 0, c->st,
                        bhp::SampleWork{dB, 1, spp, c->d_samp1}, c->d_queue, cfg.interval, cfg.refill_min, prio,
                        c->d_blk);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
@@ -1211,7 +1225,7 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
             HIPCK(c, hipMemcpyAsync(c->d_bframes + sizeof(bhp::BatchFrame), &B[1], sizeof B[1], hipMemcpyHostToDevice,
                                     c->st));
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
-            hipLaunchKernelGGL(pick_adaptive<bhp::ChunkWork>(o.precision, rk, cfg.waves), dim3(grid), dim3(cfg.block), 0, c->st,
+            hipLaunchKernelGGL(k2, dim3(grid), dim3(cfg.block), 0, c->st,
                                bhp::ChunkWork{dB + 1, 1, spp, npix, c->d_offsets, c->d_samp2}, c->d_queue,
                                cfg.interval, cfg.refill_min, prio, c->d_blk);
             hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, grid, c->d_tot);
