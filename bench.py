@@ -225,7 +225,8 @@ def main():
     results = {}
     for m in modes:
         prec, emis, persistent, f32, full = MODES[m]
-        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full)
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full,
+                           engine=bh.ENGINE_AUTO if persistent else bh.ENGINE_FRAME)
         try:
             cfg = None
             if persistent and not args.no_autotune:
@@ -241,10 +242,11 @@ def main():
             log(f"bench: {m}: not available ({e})")
 
     # accuracy of every mode against the FP64 tracer (bit-identical to the oracle) on this frame
-    ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1))
+    ref, _ = tracer.render(F, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=0, spp=1, engine=bh.ENGINE_FRAME))
     for m, r in results.items():
         prec, emis, persistent, f32, full = MODES[m]
-        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full)
+        o = bh.Tracer.opts(precision=prec, emissivity=emis, jitter=0, spp=1, far_fp32=f32, full_flow=full,
+                           engine=bh.ENGINE_AUTO if persistent else bh.ENGINE_FRAME)
         cfg = None
         if f"autotune_{m}" in setup:
             cfg = bh.BatchCfg(**setup[f"autotune_{m}"]["best"])

@@ -31,6 +31,7 @@ extern "C" {
 enum { BH_MODE_VISIBLE = 0, BH_MODE_RADIO = 1 };      /* scene.hpp:10 RenderMode */
 enum { BH_PREC_FP64 = 0, BH_PREC_MIXED = 1 };         /* plain FP64 (parity) or mixed FP32/FP64 */
 enum { BH_EMIS_EXACT = 0, BH_EMIS_MLP = 1 };          /* synchrotron coefficients: exact or surrogate */
+enum { BH_ENGINE_AUTO = 0, BH_ENGINE_FRAME = 1 };     /* kernel behind bh_render (bh_trace_opts.engine) */
 enum { BH_FATE_LIMIT = 0, BH_FATE_ESCAPED = 1, BH_FATE_CAPTURED = -1 };
 enum { BH_ORDER_ROWS = 1, BH_ORDER_IMPACT = 2, BH_ORDER_CRITICAL = 3 };   /* tile order of a persistent launch's queues */
 
@@ -114,6 +115,11 @@ typedef struct bh_trace_opts {
                          vacuum with tol_mid = max(max(1e-6, tol), tol_far / 10); 1 -> emission,
                          absorption and the step cap over the whole flow as the FP64 tracer, with
                          tol_mid = max(1e-6, tol) */
+    int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> one sample per pixel runs
+                         on the persistent engine (bh_trace_batch's kernel on a batch of one frame, with
+                         bh_autotune's last result for this precision or the defaults), more samples on
+                         the frame kernel; BH_ENGINE_FRAME -> always the frame kernel. Both give the same
+                         pixels and counters. */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

@@ -17,6 +17,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 ORDER_ROWS, ORDER_IMPACT, ORDER_CRITICAL = 1, 2, 3
 TURB_GRID, TURB_SHEARED = 0, 1
 EMIS_EXACT, EMIS_MLP = 0, 1
+ENGINE_AUTO, ENGINE_FRAME = 0, 1     # bh_render: persistent engine for one sample per pixel / frame kernel
 
 # Every symbol declared in include/bhrt.h and include/bhrt_host.h
 EXPORTS = [
@@ -70,7 +71,7 @@ class Settings(C.Structure):
 class TraceOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
                 ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int),
-                ("full_flow", C.c_int)]
+                ("full_flow", C.c_int), ("engine", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -515,8 +516,8 @@ class Tracer:
 
     @staticmethod
     def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0, r_emit=0.0,
-             far_fp32=0, full_flow=0):
-        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow)
+             far_fp32=0, full_flow=0, engine=ENGINE_AUTO):
+        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow, engine)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)

@@ -793,6 +793,7 @@ int check_opts(bh_ctx* c, const bh_trace_opts& o) {
         return fail(c, BH_ERR_ARG, "the MLP emissivity runs in the mixed-precision tracer");
     if (o.emissivity == BH_EMIS_MLP && !c->stab_ready)
         return fail(c, BH_ERR_UNSUPPORTED, "no baked surrogate: call bh_mlp_train (or bh_mlp_set) and bh_mlp_bake first");
+    if (o.engine != BH_ENGINE_AUTO && o.engine != BH_ENGINE_FRAME) return fail(c, BH_ERR_ARG, "engine must be 0 or 1");
     return BH_OK;
 }
 
@@ -971,7 +972,7 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     if (opts) o = *opts;
     int rc = check_opts(c, o);
     if (rc) return rc;
-    bh_batch_cfg cfg = cfg_in ? *cfg_in : (c->tuned_ok ? c->tuned : batch_defaults(o.precision));
+    bh_batch_cfg cfg = cfg_in ? *cfg_in : (c->tuned_ok[o.precision] ? c->tuned[o.precision] : batch_defaults(o.precision));
     rc = resolve_cfg(c, o.precision, &cfg);
     if (rc) return rc;
     const int mode = frames[0].scene.mode;
@@ -1135,7 +1136,7 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     if ((long long)tiles * 64 * spp >= (1ll << 31) ||
         (long long)npix * ((extra_max + ADAPT_CHUNK - 1) / ADAPT_CHUNK) * ADAPT_CHUNK >= (1ll << 31))
         return fail(c, BH_ERR_ARG, "frame too large for adaptive sampling (2^31 samples)");
-    bh_batch_cfg cfg = c->tuned_ok ? c->tuned : batch_defaults(o.precision);
+    bh_batch_cfg cfg = c->tuned_ok[o.precision] ? c->tuned[o.precision] : batch_defaults(o.precision);
     rc = resolve_cfg(c, o.precision, &cfg);
     if (rc) return rc;
     rc = prepare_turbulence(c, c->S, f);
@@ -1253,6 +1254,27 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
         st->flux_sum = tot1.flux + tot2.flux;
     }
     return BH_OK;
+}
+
+// A whole frame on the context's scene. One sample per pixel goes to the persistent engine (a batch
+// of one frame, launched with bh_autotune's result for this precision or the defaults): the same
+// pixels and counters as the frame kernel, without the frame kernel's per-tile tails. More samples
+// per pixel, or engine BH_ENGINE_FRAME, run the frame kernel. *d_out receives the device image.
+int render_frame(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, const float4** d_out, bh_stats* st) {
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    const int spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
+    if (o.engine == BH_ENGINE_AUTO && spp == 1) {
+        const bh_batch_frame bf{c->S, *f};
+        rc = launch_batch(c, &bf, 1, &o, nullptr, nullptr, st);
+        *d_out = c->d_bout;
+    } else {
+        rc = launch_trace(c, f, &o, 0, 0, f->W, f->H, false, st);
+        *d_out = c->d_px;
+    }
+    return rc;
 }
 
 }  // namespace
@@ -1385,10 +1407,11 @@ int bh_render(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, float* px
     double t0 = now_ms();
     HIPCK(c, hipSetDevice(c->device));
     bh_stats local{};
-    int rc = launch_trace(c, f, opts, 0, 0, f ? f->W : 0, f ? f->H : 0, false, &local);
+    const float4* img = nullptr;
+    int rc = render_frame(c, f, opts, &img, &local);
     if (rc) return rc;
     if (px_out) {
-        HIPCK(c, hipMemcpyAsync(px_out, c->d_px, sizeof(float4) * (size_t)f->W * f->H, hipMemcpyDeviceToHost, c->st));
+        HIPCK(c, hipMemcpyAsync(px_out, img, sizeof(float4) * (size_t)f->W * f->H, hipMemcpyDeviceToHost, c->st));
         HIPCK(c, hipStreamSynchronize(c->st));
     }
     local.ms_trace = now_ms() - t0;
@@ -1418,9 +1441,10 @@ int bh_render_resident(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, 
     double t0 = now_ms();
     HIPCK(c, hipSetDevice(c->device));
     bh_stats local{};
-    int rc = launch_trace(c, f, opts, 0, 0, f ? f->W : 0, f ? f->H : 0, false, &local);
+    const float4* img = nullptr;
+    int rc = render_frame(c, f, opts, &img, &local);
     if (rc) return rc;
-    if (d_px) *d_px = c->d_px;
+    if (d_px) *d_px = img;
     local.ms_trace = now_ms() - t0;
     if (st) *st = local;
     return BH_OK;
@@ -1497,7 +1521,8 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
     if (!c || !best) return BH_ERR_ARG;
     HIPCK(c, hipSetDevice(c->device));
     // coordinate descent from the defaults: one parameter at a time over its candidate list, two passes
-    bh_batch_cfg cur = batch_defaults(opts ? opts->precision : BH_PREC_FP64);
+    const int prec = opts ? opts->precision : BH_PREC_FP64;
+    bh_batch_cfg cur = batch_defaults(prec);
     int count = 0;
     double cur_ms = 0.0;
     auto time_cfg = [&](const bh_batch_cfg& g, double* ms) -> int {
@@ -1555,8 +1580,8 @@ int bh_autotune(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_o
     *best = cur;
     if (best_ms) *best_ms = cur_ms;
     if (n_tried) *n_tried = count;
-    c->tuned = cur;
-    c->tuned_ok = true;
+    c->tuned[prec] = cur;   // the precision was checked by the timed launches
+    c->tuned_ok[prec] = true;
     return BH_OK;
 }
 

@@ -102,7 +102,7 @@ struct bh_ctx {
     double4* d_samp2 = nullptr;
     size_t samp2_cap = 0;
     int n_cu = 0;
-    bh_batch_cfg tuned{};   // last bh_autotune result (used when bh_trace_batch gets no cfg)
-    bool tuned_ok = false;
+    bh_batch_cfg tuned[2]{};   // last bh_autotune result per precision (used when no cfg is given)
+    bool tuned_ok[2]{};
     std::string err;
 };

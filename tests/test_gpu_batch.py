@@ -1,8 +1,8 @@
 """Persistent batched tracer (bh_trace_batch, bh_autotune) against the one-frame-per-launch path.
 
-The persistent kernel runs the same per-ray integrator (init / step / result) as bh_render, only
-scheduled differently (work queues, lane retirement and refill, graph replay), so every pixel must
-be bit-identical to bh_render's for both precisions, for every launch configuration, and for
+The persistent kernel runs the same per-ray integrator (init / step / result) as the frame kernel
+(bh_render with engine BH_ENGINE_FRAME), only scheduled differently (work queues, lane retirement and refill, graph replay), so every pixel must
+be bit-identical to the frame kernel's for both precisions, for every launch configuration, and for
 batches that mix frame sizes, spins and turbulence time slices. Frame totals (flux, step counters)
 must agree: integer counters exactly, the flux sum to 1e-12 relative (summation order).
 """
@@ -25,11 +25,13 @@ def frame_for(S, ini, W, H, t=0.0, sets=""):
 
 
 def render_each(t, frames, opts):
-    """bh_render of each (Scene, Frame) in turn on the same context."""
+    """bh_render of each (Scene, Frame) in turn on the same context, on the frame kernel."""
     imgs, flux, cnt = [], 0.0, np.zeros(5, dtype=np.int64)
+    o = bh.TraceOpts.from_buffer_copy(opts)
+    o.engine = bh.ENGINE_FRAME
     for S, F in frames:
         t.update_scene(S)
-        img, st = t.render(F, opts)
+        img, st = t.render(F, o)
         imgs.append(img)
         flux += st.flux_sum
         cnt += [st.step_attempts, st.steps_accepted, st.flow_steps, st.step_limited, st.fp64_steps]
@@ -65,7 +67,8 @@ def test_batch_single_frame_every_config(c2, prec):
             (256, 2, 8, 16, 1, bh.ORDER_ROWS, -1), (256, 2, 8, 16, 0, bh.ORDER_IMPACT, -1),
             (64, 1, 1, 1, 0, bh.ORDER_ROWS, 64), (128, 4, 32, 48, 1, bh.ORDER_IMPACT, 128),
             (256, 8, 3, 64, 1, bh.ORDER_IMPACT, 1), (256, 4, 32, 32, 1, 0, 0),
-            (256, 4, 32, 32, 1, bh.ORDER_CRITICAL, 256), (128, 3, 16, 32, 0, bh.ORDER_CRITICAL, -1)]:
+            (256, 4, 32, 32, 1, bh.ORDER_CRITICAL, 256), (128, 3, 16, 32, 0, bh.ORDER_CRITICAL, -1),
+            (64, 32, 16, 32, 1, bh.ORDER_CRITICAL, 128)]:   # 32 per CU: more than a CU holds, capped
         got, st = t.trace_batch([(cfg["S"], cfg["F"])], o, block=block, grid_per_cu=gpc, interval=interval,
                                 refill_min=refill, use_graph=graph, order=order, prio_age=age)
         assert_same(got, want, f"prec {prec} cfg {(block, gpc, interval, refill, graph, order, age)}")
@@ -127,7 +130,7 @@ def test_graph_replay_with_eager_work_between(c2):
     frames = [(cfg["S"], cfg["F"])] * 2
     first, st0 = t.trace_batch(frames, o)
     for _ in range(4):
-        t.render(cfg["F"], t.opts())   # eager launches between replays
+        t.render(cfg["F"], t.opts(engine=bh.ENGINE_FRAME))   # eager launches between replays
         again, st = t.trace_batch(frames, o)
         assert_same(again, first, "replay")
         assert np.array_equal(stats_vec(st), stats_vec(st0))
@@ -166,6 +169,27 @@ def test_batch_errors(c2):
         t.trace_batch([(cfg["S"], cfg["F"])], order=4)
     with pytest.raises(bh.BhError, match="prio_age must be"):
         t.trace_batch([(cfg["S"], cfg["F"])], prio_age=(1 << 24) + 1)
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_render_engines_agree(c2, prec):
+    """bh_render with one sample per pixel runs on the persistent engine by default: the same pixels
+    and counters as on the frame kernel, centre and jittered; more samples per pixel run the frame
+    kernel either way."""
+    cfg, t = c2
+    t.update_scene(cfg["S"])
+    for jitter in (0, 1):
+        want, fst = t.render(cfg["F"], t.opts(precision=prec, jitter=jitter, engine=bh.ENGINE_FRAME))
+        got, ast = t.render(cfg["F"], t.opts(precision=prec, jitter=jitter))
+        assert_same([got], [want], f"engines prec {prec} jitter {jitter}")
+        assert np.array_equal(stats_vec(ast), stats_vec(fst)) and ast.samples == fst.samples == 256 * 256
+        assert abs(ast.flux_sum - fst.flux_sum) <= 1e-12 * abs(fst.flux_sum)
+    want, fst = t.render(cfg["F"], t.opts(precision=prec, jitter=1, spp=2, engine=bh.ENGINE_FRAME))
+    got, ast = t.render(cfg["F"], t.opts(precision=prec, jitter=1, spp=2))
+    assert_same([got], [want], f"spp 2 prec {prec}")
+    assert np.array_equal(stats_vec(ast), stats_vec(fst)) and ast.samples == 2 * 256 * 256
+    with pytest.raises(bh.BhError, match="engine must be"):
+        t.render(cfg["F"], t.opts(engine=2))
 
 
 @pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
