@@ -117,9 +117,9 @@ typedef struct bh_trace_opts {
                          tol_mid = max(1e-6, tol) */
     int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> one sample per pixel runs
                          on the persistent engine (bh_trace_batch's kernel on a batch of one frame, with
-                         bh_autotune's last result for this precision or the defaults), more samples on
-                         the frame kernel; BH_ENGINE_FRAME -> always the frame kernel. Both give the same
-                         pixels and counters. */
+                         bh_autotune's last result for this precision or the defaults), more samples
+                         (or frames above 2^30 pixels) on the frame kernel; BH_ENGINE_FRAME -> always
+                         the frame kernel. Both give the same pixels and counters. */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

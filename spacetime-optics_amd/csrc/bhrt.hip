@@ -1259,14 +1259,15 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
 // A whole frame on the context's scene. One sample per pixel goes to the persistent engine (a batch
 // of one frame, launched with bh_autotune's result for this precision or the defaults): the same
 // pixels and counters as the frame kernel, without the frame kernel's per-tile tails. More samples
-// per pixel, or engine BH_ENGINE_FRAME, run the frame kernel. *d_out receives the device image.
+// per pixel, frames above the batch limit of 2^30 pixels, or engine BH_ENGINE_FRAME run the frame
+// kernel. *d_out receives the device image.
 int render_frame(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, const float4** d_out, bh_stats* st) {
     int rc = check_frame(c, f);
     if (rc) return rc;
     bh_trace_opts o{};
     if (opts) o = *opts;
     const int spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
-    if (o.engine == BH_ENGINE_AUTO && spp == 1) {
+    if (o.engine == BH_ENGINE_AUTO && spp == 1 && (long long)f->W * f->H <= BATCH_MAX_PIXELS) {
         const bh_batch_frame bf{c->S, *f};
         rc = launch_batch(c, &bf, 1, &o, nullptr, nullptr, st);
         *d_out = c->d_bout;
