@@ -115,11 +115,12 @@ typedef struct bh_trace_opts {
                          vacuum with tol_mid = max(max(1e-6, tol), tol_far / 10); 1 -> emission,
                          absorption and the step cap over the whole flow as the FP64 tracer, with
                          tol_mid = max(1e-6, tol) */
-    int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> one sample per pixel runs
-                         on the persistent engine (bh_trace_batch's kernel on a batch of one frame, with
-                         bh_autotune's last result for this precision or the defaults), more samples
-                         (or frames above 2^30 pixels) on the frame kernel; BH_ENGINE_FRAME -> always
-                         the frame kernel. Both give the same pixels and counters. */
+    int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> the persistent engine
+                         (bh_autotune's last result for this precision, or the defaults): one sample per
+                         pixel as bh_trace_batch of one frame, more as the first pass of
+                         bh_render_adaptive alone; frames beyond 2^31 samples on the frame kernel.
+                         BH_ENGINE_FRAME -> always the frame kernel. Both give the same pixels and
+                         counters. */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

@@ -1120,15 +1120,17 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
 // Adaptive render (render.cu:677-721): persistent pass 1 over (tile, sample, pixel) entries, the
 // statistics and decisions on the device, one host read of the pass-2 size, persistent pass 2 over
 // chunks, resolve. px_out / nsamp_out are device pointers (d_px, d_nsamp).
-int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, long long* samples, bh_stats* st) {
+// plain_spp > 0: pass 1 only with that many samples per pixel (the plain mean of bh_render).
+int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, long long* samples, bh_stats* st,
+                    int plain_spp = 0) {
     int rc = check_frame(c, f);
     if (rc) return rc;
     bh_trace_opts o{};
     if (opts) o = *opts;
     rc = check_opts(c, o);
     if (rc) return rc;
-    const int spp = o.spp > 0 ? o.spp : (c->rs.spp > 0 ? c->rs.spp : 8);
-    const int spp_max = c->rs.spp_max > 0 ? c->rs.spp_max : 128;
+    const int spp = plain_spp > 0 ? plain_spp : (o.spp > 0 ? o.spp : (c->rs.spp > 0 ? c->rs.spp : 8));
+    const int spp_max = plain_spp > 0 ? plain_spp : (c->rs.spp_max > 0 ? c->rs.spp_max : 128);
     const float target = c->rs.target_err > 0.f ? c->rs.target_err : 0.03f;
     const int W = f->W, H = f->H, npix = W * H;
     const int extra_max = std::max(spp_max - spp, 0);
@@ -1256,21 +1258,26 @@ int launch_adaptive(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, lon
     return BH_OK;
 }
 
-// A whole frame on the context's scene. One sample per pixel goes to the persistent engine (a batch
-// of one frame, launched with bh_autotune's result for this precision or the defaults): the same
-// pixels and counters as the frame kernel, without the frame kernel's per-tile tails. More samples
-// per pixel, frames above the batch limit of 2^30 pixels, or engine BH_ENGINE_FRAME run the frame
-// kernel. *d_out receives the device image.
+// A whole frame on the context's scene, on the persistent engine (launched with bh_autotune's
+// result for this precision or the defaults): one sample per pixel as a batch of one frame, more as
+// the adaptive render's first pass alone (samples summed in index order, then divided by their
+// number). The same pixels and counters as the frame kernel, without its per-tile tails. Frames
+// beyond the engine's 2^31 work entries, or engine BH_ENGINE_FRAME, run the frame kernel.
+// *d_out receives the device image.
 int render_frame(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, const float4** d_out, bh_stats* st) {
     int rc = check_frame(c, f);
     if (rc) return rc;
     bh_trace_opts o{};
     if (opts) o = *opts;
     const int spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
+    const long long entries = (long long)((f->W + 7) / 8) * ((f->H + 7) / 8) * 64 * spp;
     if (o.engine == BH_ENGINE_AUTO && spp == 1 && (long long)f->W * f->H <= BATCH_MAX_PIXELS) {
         const bh_batch_frame bf{c->S, *f};
         rc = launch_batch(c, &bf, 1, &o, nullptr, nullptr, st);
         *d_out = c->d_bout;
+    } else if (o.engine == BH_ENGINE_AUTO && spp > 1 && entries < (1ll << 31)) {
+        rc = launch_adaptive(c, f, &o, nullptr, st, spp);
+        *d_out = c->d_px;
     } else {
         rc = launch_trace(c, f, &o, 0, 0, f->W, f->H, false, st);
         *d_out = c->d_px;

@@ -173,9 +173,9 @@ def test_batch_errors(c2):
 
 @pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
 def test_render_engines_agree(c2, prec):
-    """bh_render with one sample per pixel runs on the persistent engine by default: the same pixels
-    and counters as on the frame kernel, centre and jittered; more samples per pixel run the frame
-    kernel either way."""
+    """bh_render runs on the persistent engine by default (one sample per pixel: a batch of one
+    frame; more: the adaptive render's first pass alone): the same pixels and counters as on the
+    frame kernel, centre and jittered."""
     cfg, t = c2
     t.update_scene(cfg["S"])
     for jitter in (0, 1):
@@ -187,7 +187,8 @@ def test_render_engines_agree(c2, prec):
     want, fst = t.render(cfg["F"], t.opts(precision=prec, jitter=1, spp=2, engine=bh.ENGINE_FRAME))
     got, ast = t.render(cfg["F"], t.opts(precision=prec, jitter=1, spp=2))
     assert_same([got], [want], f"spp 2 prec {prec}")
-    assert np.array_equal(stats_vec(ast), stats_vec(fst)) and ast.samples == 2 * 256 * 256
+    assert np.array_equal(stats_vec(ast), stats_vec(fst)) and ast.samples == fst.samples == 2 * 256 * 256
+    assert abs(ast.flux_sum - fst.flux_sum) <= 1e-12 * abs(fst.flux_sum)
     with pytest.raises(bh.BhError, match="engine must be"):
         t.render(cfg["F"], t.opts(engine=2))
 
