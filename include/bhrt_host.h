@@ -74,6 +74,14 @@ int bhh_make_hud(const char* ini, const char* sets, int frame, int W, int H, int
    advance, then the glyphs x cell_h x cell_w coverage bytes). Returns 0 or -1. */
 int bhh_save_font(const char* path, const bh_font* font);
 
+/* sizeof of the public structs of bhrt.h in this order: bh_scene, bh_tables, bh_frame, bh_settings,
+   bh_trace_opts, bh_stats, bh_ray_out, bh_batch_frame, bh_batch_cfg, bh_mlp_train_cfg,
+   bh_mlp_report, bh_mlp_bake_report, bh_post_params, bh_tone_params, bh_hud_glyph, bh_hud_rect,
+   bh_font - so that a binding can check its struct layouts against the library's. No counterpart
+   in the reference (its host code shares the headers). Writes min(n, count) sizes to out
+   (nullable) and returns the count. */
+int bhh_abi_sizes(size_t* out, int n);
+
 /* Self-test rows (8 x 6: a, incl_deg, traced -, traced +, analytic -, analytic +) on `device`. */
 int bhh_selftest(int device, double* rows, char* err, size_t errlen);
 

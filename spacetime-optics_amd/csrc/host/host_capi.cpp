@@ -253,6 +253,18 @@ int bhh_make_hud(const char* ini, const char* sets, int frame, int W, int H, int
     return 0;
 }
 
+int bhh_abi_sizes(size_t* out, int n) {
+    const size_t sz[] = {sizeof(bh_scene),        sizeof(bh_tables),      sizeof(bh_frame),
+                         sizeof(bh_settings),     sizeof(bh_trace_opts),  sizeof(bh_stats),
+                         sizeof(bh_ray_out),      sizeof(bh_batch_frame), sizeof(bh_batch_cfg),
+                         sizeof(bh_mlp_train_cfg), sizeof(bh_mlp_report), sizeof(bh_mlp_bake_report),
+                         sizeof(bh_post_params),  sizeof(bh_tone_params), sizeof(bh_hud_glyph),
+                         sizeof(bh_hud_rect),     sizeof(bh_font)};
+    const int count = (int)(sizeof sz / sizeof sz[0]);
+    for (int i = 0; out && i < n && i < count; ++i) out[i] = sz[i];
+    return count;
+}
+
 int bhh_save_font(const char* path, const bh_font* font) {
     if (!path || !font || !font->atlas || font->cell_w < 1 || font->cell_h < 1 || font->glyphs < 1) return -1;
     bh::Font f;

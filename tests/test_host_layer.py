@@ -119,6 +119,22 @@ def test_c_abi_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def test_struct_layouts_match_the_library():
+    """Every ctypes mirror of a public struct has the C compiler's size (a field missing, added or
+    of the wrong type on either side changes it)."""
+    L = bh.load()
+    mirrors = [bh.Scene, bh.Tables, bh.Frame, bh.Settings, bh.TraceOpts, bh.Stats, bh.RAY_DTYPE, bh.BatchFrame,
+               bh.BatchCfg, bh.MlpTrainCfg, bh.MlpReport, bh.MlpBakeReport, bh.PostParams, bh.ToneParams,
+               bh.HudGlyph, bh.HudRect, bh.Font]
+    n = L.bhh_abi_sizes(None, 0)
+    assert n == len(mirrors)
+    sizes = (C.c_size_t * n)()
+    assert L.bhh_abi_sizes(sizes, n) == n
+    for m, got in zip(mirrors, sizes):
+        want = m.itemsize if isinstance(m, np.dtype) else C.sizeof(m)
+        assert got == want, (getattr(m, "__name__", "RAY_DTYPE"), got, want)
+
+
 def test_error_paths_without_gpu():
     S = bh.Scene()
     ctx = C.c_void_p()
