@@ -6,7 +6,8 @@
    unstable (Lyapunov factor ~e^pi per half orbit), so a one-ulp difference between two libm
    implementations grows into different step counts, winding counts and even fates. Routing every
    per-step transcendental through this header makes the GPU tracer and the CPU oracle agree bit
-   for bit; accuracy against glibc is <= a few ulp (pinned by tests/test_bh_math.py).
+   for bit; accuracy against glibc is <= 3 ulp (pinned by tests/test_bh_math.py over 10^6 inputs per
+   function), and x^-0.2 is correctly rounded.
 
    Methods (all textbook, coefficients are exact series terms rounded to double):
    - exp, expm1: Cody-Waite reduction x = k ln2 + r with a 32-bit ln2 head, |r| <= ln2/2, Taylor
@@ -19,6 +20,9 @@
      on |r| <= pi/4 (sin to r^17, cos to r^18).
    - atan2/acos: reciprocal and pi/6 (Hart) reductions to |t| <= tan(pi/12), odd Taylor series to
      t^29; acos(x) = atan2(sqrt((1-x)(1+x)), x).
+   - x^-0.2 (the step controller): exp(-0.2 log x) refined by one Newton step on x y^5 = 1 with an
+     FMA double-double residual; correctly rounded on [1e-300, 1e300]. FMA is exact on both sides
+     (v_fma_f64 on the device, the C library's fma on the host), so this stays bit-identical.
    Subnormal results of exp are flushed to zero (deterministically on both sides). */
 #pragma once
 #include <stdint.h>
@@ -149,8 +153,27 @@ BHM_HD double log(double x) {
     return ed * LN2_HI + (lm + ed * LN2_LO);
 }
 
-// x^-0.2 for x > 0 (step-size controller exponent)
-BHM_HD double pow_m02(double x) { return exp(-0.2 * log(x)); }
+// x^-0.2 for x > 0 (the step-size controller's factor, with -0.2 the double nearest -1/5), correctly
+// rounded for 1e-300 <= x <= 1e300 (checked against 200-bit arithmetic; outside that range within
+// 9 ulp). The controller's output feeds back into every later step, so the integrator's step
+// sequence, and with it every final state, follows this function's last bit: a correctly rounded
+// value agrees with glibc's pow wherever glibc's does (all but ~0.1 % of arguments).
+// Method: seed y0 = exp(-0.2 log x) (~2^-49 relative), then one Newton step on x y^5 = 1 with the
+// residual r = 1 - x y0^5 formed exactly by FMA two-products (double-double), y = y0 (1 + r/5)
+// (error 3 r^2 / 25 ~ 2^-98), times x^-(0.2 - 1/5) = 1 - (0.2 - 1/5) ln x for the double's offset
+// from -1/5 (0.2 - 1/5 = 2^-56 * 4/5 exactly).
+BHM_HD double pow_m02(double x) {
+    const double lx = log(x);
+    const double y0 = exp(-0.2 * lx);
+    if (!(x >= 1e-300 && x <= 1e300)) return y0;
+    const double p2 = y0 * y0, e2 = __builtin_fma(y0, y0, -p2);
+    const double p4 = p2 * p2, e4 = __builtin_fma(p2, p2, -p4) + (2.0 * p2) * e2;
+    const double p5 = p4 * y0, e5 = __builtin_fma(p4, y0, -p5) + e4 * y0;
+    const double p6 = p5 * x, e6 = __builtin_fma(p5, x, -p6) + e5 * x;
+    const double r = (1.0 - p6) - e6;
+    constexpr double DLT = 0x1.999999999999ap-57;   // 0.2 (double) - 1/5 = 2^-56 * 4/5, to 53 bits
+    return y0 + y0 * (r * 0.2 - DLT * lx);
+}
 
 BHM_HD double cbrt(double x) {
     if (x == 0.0 || isnan_(x) || isinf_(x)) return x;

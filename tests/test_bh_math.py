@@ -1,0 +1,135 @@
+"""Accuracy of the project's FP64 elementary functions (spacetime-optics_amd/csrc/bh_math.h) against
+glibc.
+
+The GPU tracer and the oracle's bh_math build share these functions, so the GPU/oracle parity tests
+cannot see a wrong coefficient in them; this file can. Each function is evaluated by both oracle
+builds (liboracle_bhm: bh_math.h; liboracle_libm: glibc, the reference host build's arithmetic) on
+10^6 inputs spanning its domain plus special values, and the error is measured in units of the last
+place of the glibc result. glibc's exp/log/sin/cos/atan2/acos/cbrt/expm1/pow are correctly rounded
+or within one ulp, so a bound of k ulp here means at most k + 1 ulp from the true value.
+
+Bounds asserted: exp, log <= 1 ulp; expm1, sin, cos <= 2; cbrt, atan2, acos <= 3; x^-0.2 (computed
+as exp(-0.2 log x), the step controller's factor, kerr.hpp/trace.cuh:93-94,146) <= 9, with mean
+error below 2 ulp. A dropped series term, a wrong reduction constant or a wrong quadrant fails
+these by many orders of magnitude."""
+import numpy as np
+import pytest
+
+N = 1_000_000
+
+
+def ulp_error(a, b):
+    """|a - b| in units of the spacing of the double nearest |b| (b = glibc)."""
+    sp = np.spacing(np.abs(b))
+    return np.abs(a - b) / sp
+
+
+def _inputs(rng):
+    n2 = N // 2
+    exp_x = np.concatenate([rng.uniform(-708.0, 709.7, n2), rng.uniform(-2.0, 2.0, n2)])
+    log_x = np.concatenate([np.exp(rng.uniform(-700.0, 700.0, n2)), rng.uniform(0.5, 2.0, n2)])
+    expm1_x = np.concatenate([rng.uniform(-40.0, 709.0, n2), rng.uniform(-1.0, 1.0, n2)])
+    cbrt_x = np.concatenate([np.exp(rng.uniform(-700.0, 700.0, n2)), -np.exp(rng.uniform(-700.0, 700.0, n2))])
+    # the controller's argument is max(err, 1e-30) with err a scaled error estimate
+    pow_x = np.exp(rng.uniform(np.log(1e-30), np.log(1e30), N))
+    trig_x = np.concatenate([rng.uniform(-1e4, 1e4, n2), rng.uniform(-4.0, 4.0, n2)])
+    at_y = rng.standard_normal(N) * np.exp(rng.uniform(-20.0, 20.0, N))
+    at_x = rng.standard_normal(N) * np.exp(rng.uniform(-20.0, 20.0, N))
+    acos_x = np.concatenate([rng.uniform(-1.0, 1.0, n2), 1.0 - np.exp(rng.uniform(-36.0, 0.0, n2 // 2)),
+                             -1.0 + np.exp(rng.uniform(-36.0, 0.0, n2 - n2 // 2))])
+    return {
+        "exp": (exp_x, None, 1.0),
+        "log": (log_x, None, 1.0),
+        "expm1": (expm1_x, None, 2.0),
+        "cbrt": (cbrt_x, None, 3.0),
+        "pow_m02": (pow_x, None, 9.0),
+        "sin": (trig_x, None, 2.0),
+        "cos": (trig_x, None, 2.0),
+        "atan2": (at_y, at_x, 3.0),
+        "acos": (acos_x, None, 3.0),
+    }
+
+
+@pytest.fixture(scope="module")
+def inputs():
+    return _inputs(np.random.default_rng(20240601))
+
+
+@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos"])
+def test_ulp_bound_against_glibc(fn, inputs, oracle_bhm, oracle_libm):
+    x, y, bound = inputs[fn]
+    a = oracle_bhm.math_eval(fn, x, y)
+    b = oracle_libm.math_eval(fn, x, y)
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+    e = ulp_error(a, b)
+    worst = int(np.argmax(e))
+    assert e.max() <= bound, f"{fn}: {e.max():.2f} ulp at x={x[worst]!r}" + ("" if y is None else f", y={y[worst]!r}")
+    assert e.mean() < 2.0, f"{fn}: mean {e.mean():.3f} ulp"
+
+
+def test_special_values(oracle_bhm, oracle_libm):
+    inf, nan = np.inf, np.nan
+    B = oracle_bhm.math_eval
+    # exp: overflow, underflow (flushed below the normal range), exact points
+    assert np.array_equal(B("exp", [0.0, 710.0, -746.0, -inf, inf]), [1.0, inf, 0.0, 0.0, inf])
+    assert np.isnan(B("exp", [nan]))[0]
+    assert abs(B("exp", [1.0])[0] - np.e) <= np.spacing(np.e)
+    # log: the domain edges and exact powers of two
+    lg = B("log", [1.0, 0.0, -1.0, inf, 2.0 ** -1074, 2.0 ** 1023, 0.5])
+    assert lg[0] == 0.0 and lg[1] == -inf and np.isnan(lg[2]) and lg[3] == inf
+    assert lg[4] == -1074 * np.log(2.0) and lg[5] == 1023 * np.log(2.0) and lg[6] == -np.log(2.0)
+    # expm1: identity near zero, -1 far left (the sign of a zero result is not kept: -0 -> +0)
+    em = B("expm1", [0.0, -0.0, 1e-300, -50.0])
+    assert em[0] == 0.0 and em[1] == 0.0 and em[2] == 1e-300 and em[3] == -1.0
+    # cbrt: exact cubes, sign, subnormals
+    cb = B("cbrt", [27.0, -8.0, 0.0, -0.0, 2.0 ** -1071, inf])
+    assert cb[0] == 3.0 and cb[1] == -2.0 and cb[2] == 0.0 and np.signbit(cb[3])
+    assert cb[4] == 2.0 ** -357 and cb[5] == inf
+    # x^-0.2 at exact powers (32^-0.2 = 1/2, 2^-50^-0.2 = 1024), within the 9-ulp bound
+    pm = B("pow_m02", [1.0, 32.0, 2.0 ** -50])
+    assert pm[0] == 1.0 and abs(pm[1] - 0.5) <= 9 * np.spacing(0.5) and abs(pm[2] - 1024.0) <= 9 * np.spacing(1024.0)
+    # sin/cos: zeros and quadrants
+    assert B("sin", [0.0])[0] == 0.0 and B("cos", [0.0])[0] == 1.0
+    q = np.array([np.pi / 2, np.pi, 1.5 * np.pi, 2 * np.pi])
+    assert np.allclose(B("sin", q), [1.0, 0.0, -1.0, 0.0], atol=1e-15)
+    assert np.allclose(B("cos", q), [0.0, -1.0, 0.0, 1.0], atol=1e-15)
+    assert np.isnan(B("sin", [inf]))[0] and np.isnan(B("cos", [nan]))[0]
+    # atan2: the four axes, signed zeros and the diagonals
+    ys = np.array([0.0, 1.0, 0.0, -1.0, 1.0, -0.0, 0.0, 1.0, inf])
+    xs = np.array([1.0, 0.0, -1.0, 0.0, 1.0, -1.0, -0.0, -1.0, 1.0])
+    assert np.array_equal(B("atan2", ys, xs), np.arctan2(ys, xs))
+    # acos: the ends and the middle
+    ac = B("acos", [1.0, -1.0, 0.0, 1.0 + 1e-15])
+    assert ac[0] == 0.0 and ac[1] == np.pi and ac[2] == np.pi / 2 and np.isnan(ac[3])
+
+
+def test_sqrt_and_division_correctly_rounded(oracle_bhm):
+    """sqrt and x / y are IEEE operations in both builds; checked against numpy's (also IEEE)."""
+    rng = np.random.default_rng(5)
+    x = np.exp(rng.uniform(-300, 300, 100_000))
+    y = rng.standard_normal(100_000) * np.exp(rng.uniform(-300, 300, 100_000))
+    assert np.array_equal(oracle_bhm.math_eval("sqrt", x), np.sqrt(x))
+    assert np.array_equal(oracle_bhm.math_eval("div", x, y), x / y)
+
+
+def test_identities(oracle_bhm):
+    """Relations that hold for the true functions, checked on the bh_math build alone (a wrong sign or
+    swapped branch that glibc happened to mask on the sampled inputs would show here)."""
+    rng = np.random.default_rng(9)
+    B = oracle_bhm.math_eval
+    x = rng.uniform(-20.0, 20.0, 200_000)
+    s, c = B("sin", x), B("cos", x)
+    assert np.max(np.abs(s * s + c * c - 1.0)) < 4e-16
+    e = B("exp", x)
+    assert np.max(np.abs(B("log", e) - x) / np.maximum(np.abs(x), 1.0)) < 4e-16
+    assert np.max(np.abs(B("expm1", x) - (e - 1.0)) / np.maximum(e, 1.0)) < 4e-16
+    p = np.exp(rng.uniform(-60, 60, 200_000))
+    cb = B("cbrt", p)
+    assert np.max(np.abs(cb * cb * cb / p - 1.0)) < 1e-15
+    w = B("pow_m02", p)
+    assert np.max(np.abs(w ** -5.0 / p - 1.0)) < 1e-14
+    t = rng.uniform(-np.pi, np.pi, 200_000)
+    at = B("atan2", np.sin(t), np.cos(t))
+    assert np.max(np.abs(at - t)) < 2e-15
+    u = rng.uniform(-1.0, 1.0, 200_000)
+    assert np.max(np.abs(np.cos(B("acos", u)) - u)) < 1e-15

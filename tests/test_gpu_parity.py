@@ -76,11 +76,12 @@ def test_vacuum_frame_c1_bit_exact(oracle_bhm):
 
 
 def test_vacuum_c1_against_glibc_oracle(oracle_libm):
-    """Against the glibc oracle (the reference's host arithmetic, bit for bit): fates and winding
-    counts exact on every ray. The step-size controller amplifies the last-ulp difference between
-    glibc's pow and the device x^-0.2 into shifted step placement along the same trajectory, so the
-    stopping states are compared at a common radius: cos(theta) transported to the GPU's final r
-    agrees within 1e-7, and so does the escape polar angle."""
+    """Against the glibc oracle (the reference's host arithmetic, bit for bit): fates, winding counts
+    and step counts exact on every ray. The step controller's x^-0.2 is correctly rounded on the
+    device, so it agrees with glibc's pow except where glibc misrounds (~0.1 % of arguments); each
+    such 1-ulp difference shifts the later step placement along the same trajectory, so the stopping
+    states are compared at a common radius: cos(theta) transported to the GPU's final r within 1e-10
+    and the escape polar angle within 1e-8; over three quarters of the rays end bit-identical."""
     cfg = setup(shot("c1_vacuum_256.ini"))
     t = tracer_for(cfg)
     gpu = t.trace_tile(cfg["F"], 0, 0, 256, 256).ravel()
@@ -88,13 +89,41 @@ def test_vacuum_c1_against_glibc_oracle(oracle_libm):
     ref = oracle_rays(oracle_libm, cfg, px, py)
     assert np.array_equal(gpu["fate"], ref["fate"])
     assert np.array_equal(gpu["ncross"], ref["ncross"])
-    assert (gpu["steps"] == ref["steps"]).mean() > 0.999
+    assert np.array_equal(gpu["steps"], ref["steps"])
+    ident = np.all(gpu["y"].view(np.int64) == ref["y"].view(np.int64), axis=1)
+    assert ident.mean() > 0.75
     esc = gpu["fate"] == 1
     ya, yb = gpu["y"][esc], ref["y"][esc]
     u_at_same_r = yb[:, 2] + (ya[:, 0] - yb[:, 0]) * yb[:, 3] / yb[:, 1]
-    assert np.abs(ya[:, 2] - u_at_same_r).max() < 1e-7
-    assert np.median(np.abs(ya[:, 2] - u_at_same_r)) < 1e-12
-    assert np.abs(gpu["th_inf"][esc] - ref["th_inf"][esc]).max() < 1e-7
+    assert np.abs(ya[:, 2] - u_at_same_r).max() < 1e-10
+    assert np.median(np.abs(ya[:, 2] - u_at_same_r)) == 0.0
+    assert np.abs(gpu["th_inf"][esc] - ref["th_inf"][esc]).max() < 1e-8
+
+
+def test_radio_c2_256_against_glibc_oracle(oracle_libm):
+    """The 230 GHz radio frame at 256^2 against the glibc oracle, every ray: fates, winding counts
+    and ring orders exact; step counts on all but a few rays (glibc's misrounded pow, see above);
+    frame flux within 1e-8 (the bar is 1e-6); per-pixel intensity median 0, p99 below 1e-7, worst
+    below 1e-5 of the pixel (pixels above 1e-3 of the peak); escape directions within 1e-9 for
+    99.9 % of escaped rays."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=256, H=256)
+    t = tracer_for(cfg)
+    gpu = t.trace_tile(cfg["F"], 0, 0, 256, 256).ravel()
+    px, py = pixel_grid(256, 256)
+    ref = oracle_rays(oracle_libm, cfg, px, py)
+    for f in ("fate", "ncross", "order"):
+        assert np.array_equal(gpu[f], ref[f]), f
+    assert (gpu["steps"] != ref["steps"]).sum() <= 6
+    ident = np.all(gpu["y"].view(np.int64) == ref["y"].view(np.int64), axis=1)
+    assert ident.mean() > 0.75
+    Ig, Ir = gpu["I"][:, 0], ref["I"][:, 0]
+    assert abs(Ig.sum() / Ir.sum() - 1.0) < 1e-8
+    m = Ir > 1e-3 * Ir.max()
+    rel = np.abs(Ig[m] - Ir[m]) / Ir[m]
+    assert np.median(rel) < 1e-14 and np.quantile(rel, 0.99) < 1e-7 and rel.max() < 1e-5
+    esc = ref["fate"] == 1
+    dth = np.abs(gpu["th_inf"][esc] - ref["th_inf"][esc])
+    assert np.quantile(dth, 0.999) < 1e-9 and dth.max() < 1e-6
 
 
 def test_radio_tiles_c2_bit_exact(oracle_bhm):
