@@ -174,6 +174,18 @@ int bh_render(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, flo
 int bh_render_resident(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, const void** d_px,
                        bh_stats* stats);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc): bh_render_resident on the
+   frame kernel (the BH_ENGINE_FRAME path, same pixels and counters), writing the frame into the
+   caller's device buffer d_px (W*H float4 = W*H*4 floats) and allocating nothing. The frame kernel's
+   per-workgroup partials live in a fixed device array of the library (3 MiB per device, held until
+   the process ends), so frames are limited to 65536 workgroups of 16 x 16 pixels (4096 x 4096);
+   larger frames fail with BH_ERR_ARG. Every call on one device shares that array: calls that overlap
+   on one device (several contexts or threads) race and must be serialised by the caller. A context
+   with sheared turbulence must have traced one frame (any entry point) before, so the slice buffers
+   exist; else BH_ERR_UNSUPPORTED. Synchronous, like bh_render. */
+int bh_render_noalloc(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* opts, void* d_px, bh_stats* stats);
+
 /* Tile interface (k_pass1's per-pixel launch, render.cu:89-128, with every per-ray output): rays of
    pixels x0 <= px < x0+w, y0 <= py < y0+h, one sample each (centre or jittered sample 0), written
    row-major to out[w*h]. */

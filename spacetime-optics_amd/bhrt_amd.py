@@ -27,7 +27,7 @@ EXPORTS = [
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
     "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap", "bh_sheared_slices",
-    "bh_tonemap_hud", "bhh_make_hud", "bhh_save_font", "bhh_abi_sizes",
+    "bh_tonemap_hud", "bhh_make_hud", "bhh_save_font", "bhh_abi_sizes", "bh_render_noalloc",
 ]
 MLP_NPARAM = 8706
 SHOT_VALUES = 20   # values per frame of bhh_build_shot
@@ -182,6 +182,8 @@ def load():
     L.bh_render_resident.restype = C.c_int
     L.bh_render_resident.argtypes = [C.c_void_p, C.POINTER(Frame), C.POINTER(TraceOpts), C.POINTER(C.c_void_p),
                                      C.POINTER(Stats)]
+    L.bh_render_noalloc.restype = C.c_int
+    L.bh_render_noalloc.argtypes = [C.c_void_p, C.POINTER(Frame), C.POINTER(TraceOpts), C.c_void_p, C.POINTER(Stats)]
     L.bh_trace_tile.restype = C.c_int
     L.bh_trace_tile.argtypes = [C.c_void_p, C.POINTER(Frame), C.c_int, C.c_int, C.c_int, C.c_int,
                                 C.POINTER(TraceOpts), C.c_void_p]
@@ -547,6 +549,15 @@ class Tracer:
         self._check(self.L.bh_render_resident(self.ctx, C.byref(F), C.byref(o), C.byref(ptr), C.byref(st)),
                     "bh_render_resident")
         return ptr.value, st
+
+    def render_noalloc(self, F, d_px, opts=None):
+        """bh_render_noalloc: the frame into the caller's device buffer d_px (an address of W*H*4
+        floats); returns the Stats."""
+        st = Stats()
+        o = opts or self.opts()
+        self._check(self.L.bh_render_noalloc(self.ctx, C.byref(F), C.byref(o), C.c_void_p(d_px), C.byref(st)),
+                    "bh_render_noalloc")
+        return st
 
     def trace_tile(self, F, x0, y0, w, h, opts=None):
         out = np.zeros(w * h, dtype=RAY_DTYPE)

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -154,6 +155,19 @@ __global__ __launch_bounds__(256) void k_reduce(const BlockOut* __restrict__ blk
         *tot = b;
     }
 }
+
+}  // namespace
+
+/* Scratch memory of bh_render_noalloc: the frame kernel's per-workgroup partials followed by the
+   reduced total, for frames of up to NOALLOC_MAX_BLOCKS workgroups (16 x 16 tiles, padded to a
+   multiple of the 8 XCDs): 65536 x 48 B = 3 MiB per device, held until the process ends. Host code
+   never uses this name as an address: bh_create resolves each device's copy with
+   hipGetSymbolAddress and keeps the pointer per device (g_noalloc_scratch). */
+constexpr int NOALLOC_MAX_BLOCKS = 65536;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) BlockOut scratch[NOALLOC_MAX_BLOCKS + 1];
+
+namespace {
 
 // WAVES > 1 asks the compiler for at least that many resident waves per SIMD (fewer VGPRs, possibly
 // spills); 1 leaves the register budget to the compiler.
@@ -484,6 +498,10 @@ double now_ms() {
 
 std::string g_create_error;
 
+// Device address of `scratch` on each device (resolved once per device by bh_create)
+BlockOut* g_noalloc_scratch[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_noalloc_mu;
+
 }  // namespace
 
 
@@ -583,7 +601,8 @@ TraceParams make_params(const bh_ctx* c, const bh_frame* f) { return make_params
 
 // Sheared turbulence of a radio scene seen from frame f: bakes the slices (render.cu:594-619,
 // 679-682) unless the ones in memory belong to the same noise parameters and camera.
-int prepare_turbulence(bh_ctx* c, const bh_scene& S, const bh_frame* f) {
+// With may_alloc false the slice buffers must already hold this bake (BH_ERR_UNSUPPORTED otherwise).
+int prepare_turbulence(bh_ctx* c, const bh_scene& S, const bh_frame* f, bool may_alloc = true) {
     if (c->turb_model != BH_TURB_SHEARED || !f || S.mode != BH_MODE_RADIO || !S.flow_on || !(S.flow_turb_amp > 0.f))
         return BH_OK;
     if (!(S.flow_turb_scale > 0.f) || !(S.flow_tcorr > 0.f))
@@ -612,6 +631,9 @@ int prepare_turbulence(bh_ctx* c, const bh_scene& S, const bh_frame* f) {
                                         (double)B.nlr, (double)B.nz, (double)B.nt, 1.0};
     if (c->ts_valid && key == c->ts_key) return BH_OK;
     const unsigned long long total = (unsigned long long)B.nphi * B.nlr * B.nz * B.nt;
+    if (!may_alloc && (!c->d_ts || c->ts_cap < total || !c->d_tsv))
+        return fail(c, BH_ERR_UNSUPPORTED,
+                    "sheared turbulence slices are not allocated yet: trace one frame with bh_render first");
     c->ts_valid = false;
     int rc = ensure(c, &c->d_ts, &c->ts_cap, (size_t)total);
     if (rc) return rc;
@@ -680,10 +702,10 @@ double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2
     const int n = 2000;
     const double r0 = S.rh * 1.001, r1 = S.r_flow_max;
     if (!(r1 > r0) || k2.size() < 2) return r1;
-    std::vector<double> w(n), alpha(n);
-    double tot = 0.0;
     const double dr = (r1 - r0) / n;
-    for (int i = 0; i < n; ++i) {
+    // emission weight j r^2 and absorption coefficient of radial cell i (no storage: the bh_render_noalloc
+    // path must not allocate, so the two passes below evaluate each cell twice)
+    auto cell = [&](int i, double* w, double* alpha) {
         const double r = r0 + dr * (i + 0.5);
         const double ne = S.n0 * std::pow(r, -S.n_index) * (1.0 - std::exp(-(r - S.rh) * (r - S.rh) / 0.25));
         const double th = S.theta0 * std::pow(r, -S.theta_index);
@@ -700,17 +722,22 @@ double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2
                          std::exp(-x3);
         const double bnu = 2.0 * C_H * S.nu_obs * S.nu_obs * S.nu_obs / (C_C * C_C) /
                            std::expm1(C_H * S.nu_obs / (C_ME * C_C * C_C * th));
-        w[i] = std::isfinite(j) ? j * r * r : 0.0;
-        alpha[i] = std::isfinite(j) && bnu > 0.0 ? j / bnu : 0.0;
-        tot += w[i];
+        *w = std::isfinite(j) ? j * r * r : 0.0;
+        *alpha = std::isfinite(j) && bnu > 0.0 ? j / bnu : 0.0;
+    };
+    double tot = 0.0, w, alpha;
+    for (int i = 0; i < n; ++i) {
+        cell(i, &w, &alpha);
+        tot += w;
     }
     if (!(tot > 0.0)) return r1;
     // outermost radius where the emission outside still exceeds frac of the total, or the mid-plane
     // radial optical depth from there to the flow's edge exceeds tau_max
     double out = 0.0, tau = 0.0;
     for (int i = n - 1; i >= 0; --i) {
-        out += w[i];
-        tau += alpha[i] * dr * S.rg_cm;
+        cell(i, &w, &alpha);
+        out += w;
+        tau += alpha * dr * S.rg_cm;
         if (out > frac * tot || tau > tau_max) return r0 + dr * (i + 1.0);
     }
     return r0;
@@ -797,22 +824,26 @@ int check_opts(bh_ctx* c, const bh_trace_opts& o) {
     return BH_OK;
 }
 
-// Launches the trace of a w x h tile at (x0, y0) and the block reduction; fills stats counters.
-int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0, int y0, int w, int h, bool rays,
-                 bh_stats* st) {
-    int rc = check_frame(c, f);
-    if (rc) return rc;
-    if (w <= 0 || h <= 0) return fail(c, BH_ERR_ARG, "empty tile");
-    bh_trace_opts o{};
-    if (opts) o = *opts;
-    rc = check_opts(c, o);
-    if (rc) return rc;
+// Workgroups of the frame kernel for a w x h tile: 16 x 16 tiles padded to a multiple of the XCDs.
+int trace_blocks(int w, int h, int* per_xcd) {
+    const int ntiles = ((w + TILE - 1) / TILE) * ((h + TILE - 1) / TILE);
+    *per_xcd = (ntiles + NXCD - 1) / NXCD;
+    return *per_xcd * NXCD;
+}
+
+// Frame-kernel launch of a w x h tile at (x0, y0) into the given device buffers (px or rays, the
+// block partials blk[nb] and the total), then the block reduction; fills the stats counters.
+// Allocates nothing: the callers provide every buffer, and the sheared turbulence slices must already
+// have room for this frame's bake when may_alloc is false.
+int launch_trace_into(bh_ctx* c, const bh_frame* f, const bh_trace_opts& o, int x0, int y0, int w, int h, bool rays,
+                      float4* px, bh_ray_out* rays_out, BlockOut* blk, BlockOut* tot_dev, bool may_alloc,
+                      bh_stats* st) {
     bhm32::MixedParams M = make_mixed(c, c->S, f, o);
     if (o.emissivity == BH_EMIS_MLP) {
         M.use_surrogate = 1;
         M.S = c->stab;
     }
-    rc = prepare_turbulence(c, c->S, f);
+    int rc = prepare_turbulence(c, c->S, f, may_alloc);
     if (rc) return rc;
     TraceParams P = make_params(c, f);
     P.x0 = x0;
@@ -822,23 +853,17 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
     P.spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
     if (rays) P.spp = 1;
     P.jitter = o.jitter;
-    const int ntiles = ((w + TILE - 1) / TILE) * ((h + TILE - 1) / TILE);
-    const int per_xcd = (ntiles + NXCD - 1) / NXCD;
-    const int nb = per_xcd * NXCD;
-    rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)nb);
-    if (rc) return rc;
-    if (rays) rc = ensure(c, &c->d_rays, &c->rays_cap, (size_t)w * h);
-    else rc = ensure(c, &c->d_px, &c->px_cap, (size_t)w * h);
-    if (rc) return rc;
+    int per_xcd;
+    const int nb = trace_blocks(w, h, &per_xcd);
     KernelFn k = pick_kernel(o.precision, radio_kind(c, c->S.mode), rays);
     HIPCK(c, hipEventRecord(c->ev0, c->st));
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, M, c->d_px, c->d_rays, c->d_blk, per_xcd);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, c->st, P, M, px, rays_out, blk, per_xcd);
     HIPCK(c, hipGetLastError());
     HIPCK(c, hipEventRecord(c->ev1, c->st));
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, c->d_blk, nb, c->d_tot);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->st, blk, nb, tot_dev);
     HIPCK(c, hipGetLastError());
     BlockOut tot;
-    HIPCK(c, hipMemcpyAsync(&tot, c->d_tot, sizeof tot, hipMemcpyDeviceToHost, c->st));
+    HIPCK(c, hipMemcpyAsync(&tot, tot_dev, sizeof tot, hipMemcpyDeviceToHost, c->st));
     HIPCK(c, hipStreamSynchronize(c->st));
     if (st) {
         float ms = 0.f;
@@ -853,6 +878,27 @@ int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0
         st->flux_sum = tot.flux;
     }
     return BH_OK;
+}
+
+// Launches the trace of a w x h tile at (x0, y0) and the block reduction into the context's buffers;
+// fills stats counters.
+int launch_trace(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0, int y0, int w, int h, bool rays,
+                 bh_stats* st) {
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0) return fail(c, BH_ERR_ARG, "empty tile");
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    rc = check_opts(c, o);
+    if (rc) return rc;
+    int per_xcd;
+    const int nb = trace_blocks(w, h, &per_xcd);
+    rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)nb);
+    if (rc) return rc;
+    if (rays) rc = ensure(c, &c->d_rays, &c->rays_cap, (size_t)w * h);
+    else rc = ensure(c, &c->d_px, &c->px_cap, (size_t)w * h);
+    if (rc) return rc;
+    return launch_trace_into(c, f, o, x0, y0, w, h, rays, c->d_px, c->d_rays, c->d_blk, c->d_tot, true, st);
 }
 
 using PersistFn = void (*)(const bhp::BatchWork, unsigned int*, int, int, int, BlockOut*);
@@ -1341,6 +1387,19 @@ int bh_create(int device, const bh_scene* scene, const bh_tables* tables, const 
         c->err = "cannot query the compute-unit count";
         return bail(BH_ERR_HIP);
     }
+    c->emit_cache.reserve(64);   // make_mixed then never allocates (bh_render_noalloc)
+    if (device < NOALLOC_MAX_DEVICES) {
+        // this device's copy of the no-alloc scratch array, resolved once with the device current
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_scratch[device]) {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                c->err = "hipGetSymbolAddress of the no-alloc scratch array failed";
+                return bail(BH_ERR_HIP);
+            }
+            g_noalloc_scratch[device] = static_cast<BlockOut*>(ptr);
+        }
+    }
     if (tables) {
         if (tables->k2_ln && tables->k2_n >= 2) {
             c->k2_n = tables->k2_n;
@@ -1453,6 +1512,34 @@ int bh_render_resident(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, 
     int rc = render_frame(c, f, opts, &img, &local);
     if (rc) return rc;
     if (d_px) *d_px = img;
+    local.ms_trace = now_ms() - t0;
+    if (st) *st = local;
+    return BH_OK;
+}
+
+int bh_render_noalloc(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, void* d_px, bh_stats* st) {
+    if (!c) return BH_ERR_ARG;
+    double t0 = now_ms();
+    HIPCK(c, hipSetDevice(c->device));
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    if (!d_px) return fail(c, BH_ERR_ARG, "bh_render_noalloc: d_px is NULL");
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    rc = check_opts(c, o);
+    if (rc) return rc;
+    int per_xcd;
+    const int nb = trace_blocks(f->W, f->H, &per_xcd);   // W, H <= 65536: no overflow
+    if (nb > NOALLOC_MAX_BLOCKS)
+        return fail(c, BH_ERR_ARG, "bh_render_noalloc: the frame needs " + std::to_string(nb) +
+                                       " workgroups of 16x16 pixels; the scratch array holds " +
+                                       std::to_string(NOALLOC_MAX_BLOCKS) + " (e.g. 4096 x 4096 pixels)");
+    BlockOut* s = c->device < NOALLOC_MAX_DEVICES ? g_noalloc_scratch[c->device] : nullptr;
+    if (!s) return fail(c, BH_ERR_UNSUPPORTED, "bh_render_noalloc: no scratch array on this device");
+    bh_stats local{};
+    rc = launch_trace_into(c, f, o, 0, 0, f->W, f->H, false, static_cast<float4*>(d_px), nullptr, s, s + nb, false,
+                           &local);
+    if (rc) return rc;
     local.ms_trace = now_ms() - t0;
     if (st) *st = local;
     return BH_OK;
