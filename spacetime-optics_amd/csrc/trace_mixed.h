@@ -80,10 +80,18 @@ __device__ __forceinline__ double rcp<double>(double x) { return rcpd(x); }
 __device__ __forceinline__ float lnf(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
 __device__ __forceinline__ float expf_(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
-// x^-0.2 to FP32 accuracy (step-size controller factor), x > 0
-__device__ __forceinline__ double pow_m02_fast(double x) {
-    float f = (float)fmin(fmax(x, 1e-30), 1e30);
-    return (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(f));
+// x^-0.2 in FP64 (the step-size controller's factor) for x clamped to [1e-30, 1e30]: an FP32 seed from
+// the hardware log2 / exp2 (relative error ~1e-6), then two Newton steps on x y^5 = 1,
+// y <- y + y (1 - x y^5) / 5, each squaring the relative error (times 2): 1e-6 -> 2e-12 -> FP64 rounding.
+// A NaN x gives the factor of 1e-30 (the callers handle a NaN error themselves).
+__device__ __forceinline__ double pow_m02_d(double x) {
+    const double xc = fmin(fmax(x, 1e-30), 1e30);
+    double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xc));
+    for (int i = 0; i < 2; ++i) {
+        const double y2 = y * y, y5 = y2 * y2 * y;
+        y = fma(y * 0.2, fma(-xc, y5, 1.0), y);
+    }
+    return y;
 }
 
 template <typename R>
@@ -174,18 +182,17 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     o.k4 = (double)k.d4;
     o.k5 = (double)k.d5;
     E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
-    // scaled max-norm (kerr.hpp:145-155) in FP32: the norm only decides acceptance and the next step
-    // size (itself computed in FP32), so 24 bits are ample; the scales are >= tol * 1e-6
-    const float tf = (float)tol, hf = (float)h;
-    const float sc0 = tf * (1e-6f + (float)fmax(fabs(s.y0), fabs(o.y0)));
-    const float sc1 = tf * (1e-6f + (float)fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
-    const float sc3 = tf * fmaxf(1.f, (float)fmax(fabs(s.y3), fabs(o.y3)));
-    float m = fabsf(hf * (float)E0) * rcpf(sc0);
-    m = fmaxf(m, fabsf(hf * (float)E1) * rcpf(sc1));
-    m = fmaxf(m, fabsf(hf * (float)E2) * rcpf(tf));
-    m = fmaxf(m, fabsf(hf * (float)E3) * rcpf(sc3));
-    m = fmaxf(m, fabsf(hf * (float)E4) * rcpf((float)tol4));
-    return (double)m;
+    // scaled max-norm over r, r', u, u', phi (kerr.hpp:145-155) in FP64, with Newton-refined FP64
+    // reciprocals of the scales in place of the divisions
+    const double sc0 = tol * (1e-6 + fmax(fabs(s.y0), fabs(o.y0)));
+    const double sc1 = tol * (1e-6 + fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
+    const double sc3 = tol * fmax(1.0, fmax(fabs(s.y3), fabs(o.y3)));
+    double m = fabs(h * (double)E0) * rcpd(sc0);
+    m = fmax(m, fabs(h * (double)E1) * rcpd(sc1));
+    m = fmax(m, fabs(h * (double)E2) * rcpd(tol));
+    m = fmax(m, fabs(h * (double)E3) * rcpd(sc3));
+    m = fmax(m, fabs(h * (double)E4) * rcpd(tol4));
+    return m;
 }
 
 template <typename R>
@@ -488,7 +495,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     R.k1_f32 = R.s.y0 >= M.r_f32;
     ++R.it;
     if (!(err <= 1.0)) {
-        R.h = hc * fmax(0.2, 0.9 * pow_m02_fast(err));
+        R.h = hc * fmax(0.2, 0.9 * pow_m02_d(err));
         if (!(err == err)) R.h = hc * 0.1;
         if (R.it >= P.max_steps) R.done = true;
         return;
@@ -543,7 +550,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     const double r0 = R.s.y0;
     R.s = n;
     bhd::project_radial(P, R.c, R.s);
-    double fac = fmin(5.0, fmax(0.2, 0.9 * pow_m02_fast(err)));
+    double fac = fmin(5.0, fmax(0.2, 0.9 * pow_m02_d(err)));
     // Outside the scene-tolerance region the step follows the radial scale: there the Mino-time
     // step that meets the tolerance goes as 1/r (dr/dlambda ~ r^2), and the error-only controller
     // lags one step behind it - on the way out every other step was rejected.
