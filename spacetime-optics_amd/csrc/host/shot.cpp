@@ -28,37 +28,80 @@ V3 rotate(V3 v, V3 axis, double ang) {
     return v * cs + cross(axis, v) * sn + axis * (dot(axis, v) * (1 - cs));
 }
 
-bool parse_track(const std::string& s, bool log, Track* tr, std::string* err) {
-    std::stringstream ss(s);
-    std::string item;
-    while (std::getline(ss, item, ',')) {
-        item = trim(item);
-        if (item.empty()) continue;
-        size_t c = item.find(':');
-        if (c == std::string::npos) {
-            if (err) *err = "track entry '" + item + "' needs time:value";
+// Reads "t0:v0, t1:v1, ..." (seconds : value) into a track; scale-like quantities (log) interpolate
+// ln v. Empty entries are skipped. False, with the reason, when an entry is not two numbers joined by
+// ':' or when there is no entry at all.
+bool read_track(const std::string& text, bool log, Track* tr, std::string* why) {
+    tr->t.clear();
+    tr->v.clear();
+    tr->log = log;
+    for (size_t pos = 0; pos <= text.size();) {
+        size_t comma = text.find(',', pos);
+        if (comma == std::string::npos) comma = text.size();
+        const std::string entry = trim(text.substr(pos, comma - pos));
+        pos = comma + 1;
+        if (entry.empty()) continue;
+        const size_t colon = entry.find(':');
+        if (colon == std::string::npos) {
+            *why = "'" + entry + "' is not time:value";
             return false;
         }
-        double t = std::strtod(item.substr(0, c).c_str(), nullptr);
-        double v = std::strtod(item.substr(c + 1).c_str(), nullptr);
-        tr->t.push_back(t);
+        tr->t.push_back(std::strtod(entry.c_str(), nullptr));   // stops at the ':'
+        const double v = std::strtod(entry.c_str() + colon + 1, nullptr);
         tr->v.push_back(log ? std::log(v) : v);
     }
-    tr->log = log;
+    if (tr->t.empty()) *why = "no time:value entries";
     return !tr->t.empty();
 }
 
-// value of `key` at shot time `time`: the [track] entry if it parses, else the constant (shot.cpp:29-41)
-double param(const Config& c, const std::string& key, double def, double time, bool log) {
+// One timeline quantity of a shot, resolved once: its [track] curve when "track.<key>" holds a valid
+// one (an invalid one is reported and ignored), else the key's constant (or the default).
+struct Channel {
+    bool tracked = false;
+    double constant = 0.0;
+    Track curve;
+    double at(double time) const { return tracked ? curve.value(time) : constant; }
+};
+
+Channel resolve(const Config& c, const std::string& key, double dflt, bool log) {
+    Channel ch;
     const std::string tk = "track." + key;
     if (c.has(tk)) {
-        Track tr;
-        std::string err;
-        if (parse_track(c.str(tk), log, &tr, &err)) return tr.value(time);
-        std::fprintf(stderr, "warning: %s\n", err.c_str());
+        std::string why;
+        ch.tracked = read_track(c.str(tk), log, &ch.curve, &why);
+        if (!ch.tracked) std::fprintf(stderr, "warning: [track] %s: %s; using the constant\n", key.c_str(), why.c_str());
     }
-    return c.num(key, def);
+    if (!ch.tracked) ch.constant = c.num(key, dflt);
+    return ch;
 }
+
+// The per-frame quantities that are plain channels: key, default, interpolated in log space, given
+// in degrees (stored in radians), destination (shot.cpp:78-99 reads the same keys and defaults).
+struct ChannelSpec {
+    const char* key;
+    double dflt;
+    bool log, degrees;
+    double ShotFrame::*field;
+};
+const ChannelSpec TIMELINE[] = {
+    {"camera.r", 1e6, true, false, &ShotFrame::r},
+    {"camera.incl", 80, false, true, &ShotFrame::incl},
+    {"camera.phi", 0, false, true, &ShotFrame::phi},
+    {"camera.roll", 0, false, true, &ShotFrame::roll},
+    {"camera.yaw", 0, false, true, &ShotFrame::yaw},
+    {"camera.pitch", 0, false, true, &ShotFrame::pitch},
+    {"camera.zoom", 1, true, false, &ShotFrame::zoom},
+    {"camera.target", 0, false, false, &ShotFrame::target},
+    {"camera.offset_x_m", 0, false, false, &ShotFrame::off_x},
+    {"camera.offset_y_m", 0, false, false, &ShotFrame::off_y},
+    {"time.t_m", 0, false, false, &ShotFrame::time_m},
+    // post-processing timeline: beam blur, order tint, direct-image removal, exposure offset, HUD opacity
+    {"post.blur", 0, false, false, &ShotFrame::blur},
+    {"post.overlay", 0, false, false, &ShotFrame::overlay},
+    {"post.hide_direct", 0, false, false, &ShotFrame::hide_direct},
+    {"post.exposure_ev", 0, false, false, &ShotFrame::exposure_ev},
+    {"hud.alpha", 1, false, false, &ShotFrame::hud},
+};
 
 }  // namespace
 
@@ -87,7 +130,9 @@ double Track::value(double time) const {
     return log ? std::exp(out) : out;
 }
 
-// shot.cpp:73-114 (the camera and time parameters; post/HUD tracks are out of scope)
+// The shot's frame list (shot.cpp:73-114): frame i at t = i / fps, every quantity sampled from its
+// channel; the field of view in M (camera.fov_m) or degrees (camera.fov_deg); the fade is post.fade
+// times linear ramps over shot.fade_in and shot.fade_out seconds at the two ends.
 bool build_shot(const Config& c, Shot* shot, std::string* err) {
     shot->name = c.str("shot.name", "shot");
     shot->type = c.str("shot.type", "render");
@@ -98,41 +143,33 @@ bool build_shot(const Config& c, Shot* shot, std::string* err) {
         if (err) *err = "shot.fps must be > 0";
         return false;
     }
-    int n = std::max(1, (int)std::lround(shot->duration * shot->fps));
-    if (c.has("shot.frames")) n = c.integer("shot.frames", n);
-    const double D2R = PI / 180;
-    shot->frames.clear();
-    for (int i = 0; i < n; ++i) {
-        ShotFrame f;
+    const int by_time = std::max(1, (int)std::lround(shot->duration * shot->fps));
+    const int count = c.has("shot.frames") ? c.integer("shot.frames", by_time) : by_time;
+    const double deg = PI / 180;
+    constexpr size_t NCH = sizeof(TIMELINE) / sizeof(TIMELINE[0]);
+    std::vector<Channel> chans;
+    chans.reserve(NCH);
+    for (const ChannelSpec& sp : TIMELINE) chans.push_back(resolve(c, sp.key, sp.dflt, sp.log));
+    const bool fov_in_m = c.has("camera.fov_m") || c.has("track.camera.fov_m");
+    const Channel fov = fov_in_m ? resolve(c, "camera.fov_m", 30, true) : resolve(c, "camera.fov_deg", 40, true);
+    const Channel fade = resolve(c, "post.fade", 1, false);
+    const double fade_in = c.num("shot.fade_in", 0.0), fade_out = c.num("shot.fade_out", 0.0);
+
+    shot->frames.assign((size_t)std::max(count, 0), ShotFrame{});
+    for (int i = 0; i < count; ++i) {
+        ShotFrame& f = shot->frames[(size_t)i];
         f.index = i;
         f.t = i / shot->fps;
-        const double t = f.t;
-        f.r = param(c, "camera.r", 1e6, t, true);
-        f.incl = param(c, "camera.incl", 80, t, false) * D2R;
-        f.phi = param(c, "camera.phi", 0, t, false) * D2R;
-        f.roll = param(c, "camera.roll", 0, t, false) * D2R;
-        f.yaw = param(c, "camera.yaw", 0, t, false) * D2R;
-        f.pitch = param(c, "camera.pitch", 0, t, false) * D2R;
-        f.fov_in_m = c.has("camera.fov_m") || c.has("track.camera.fov_m");
-        f.fov = f.fov_in_m ? param(c, "camera.fov_m", 30, t, true) : param(c, "camera.fov_deg", 40, t, true) * D2R;
-        f.zoom = param(c, "camera.zoom", 1, t, true);
-        f.target = param(c, "camera.target", 0, t, false);
-        f.off_x = param(c, "camera.offset_x_m", 0, t, false);
-        f.off_y = param(c, "camera.offset_y_m", 0, t, false);
-        f.time_m = param(c, "time.t_m", 0, t, false);
-        // post-processing timeline (shot.cpp:89-99): beam blur, order tint, direct-image removal,
-        // exposure offset, fades, HUD opacity
-        f.blur = param(c, "post.blur", 0, t, false);
-        f.overlay = param(c, "post.overlay", 0, t, false);
-        f.hide_direct = param(c, "post.hide_direct", 0, t, false);
-        f.exposure_ev = param(c, "post.exposure_ev", 0, t, false);
-        const double fade_in = c.num("shot.fade_in", 0.0), fade_out = c.num("shot.fade_out", 0.0);
-        double fade = param(c, "post.fade", 1, t, false);
-        if (fade_in > 0) fade *= std::min(1.0, t / fade_in);
-        if (fade_out > 0) fade *= std::min(1.0, (shot->duration - t) / fade_out);
-        f.fade = std::max(0.0, fade);
-        f.hud = param(c, "hud.alpha", 1, t, false);
-        shot->frames.push_back(f);
+        for (size_t k = 0; k < NCH; ++k) {
+            const double v = chans[k].at(f.t);
+            f.*(TIMELINE[k].field) = TIMELINE[k].degrees ? v * deg : v;
+        }
+        f.fov_in_m = fov_in_m;
+        f.fov = fov_in_m ? fov.at(f.t) : fov.at(f.t) * deg;
+        double level = fade.at(f.t);
+        if (fade_in > 0) level *= std::min(1.0, f.t / fade_in);
+        if (fade_out > 0) level *= std::min(1.0, (shot->duration - f.t) / fade_out);
+        f.fade = std::max(0.0, level);
     }
     return true;
 }

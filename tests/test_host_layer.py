@@ -59,9 +59,8 @@ def test_config_grammar(golden):
         assert (got if got is not None else "<absent>") == str(v), k
 
 
-def test_config_errors():
-    bad = os.path.join(ROOT, "_work", "bad_section.ini")
-    os.makedirs(os.path.dirname(bad), exist_ok=True)
+def test_config_errors(tmp_path):
+    bad = str(tmp_path / "bad_section.ini")
     with open(bad, "w") as f:
         f.write("[render\nmode = radio\n")
     with pytest.raises(bh.BhError, match="bad section header"):
