@@ -35,8 +35,10 @@ enum { BH_ENGINE_AUTO = 0, BH_ENGINE_FRAME = 1 };     /* kernel behind bh_render
 enum { BH_FATE_LIMIT = 0, BH_FATE_ESCAPED = 1, BH_FATE_CAPTURED = -1 };
 enum { BH_ORDER_ROWS = 1, BH_ORDER_IMPACT = 2, BH_ORDER_CRITICAL = 3 };   /* tile order of a persistent launch's queues */
 
-/* SceneGPU (scene.hpp:12-69): the scalars on the vacuum/radio path. The thin disk, sky and colour
-   fields are out of scope (visible-mode presentation) and absent. */
+/* SceneGPU (scene.hpp:12-69): the scalars on the vacuum/radio path, and the celestial sphere of the
+   visible mode. The thin disk and colour fields are out of scope (visible-mode presentation) and
+   absent. The reference samples a star-map texture (sky_tex, scene.hpp:58-61); this path's sky is a
+   procedural checkerboard on the same galactic texture coordinates (trace.cuh:25-33). */
 typedef struct bh_scene {
     double a, rh, r_isco;
     double tol;             /* DP54 tolerance (render.tolerance) */
@@ -49,6 +51,10 @@ typedef struct bh_scene {
     uint32_t seed;
     float flow_turb_scale;  /* sheared turbulence: feature size in units of R (flow.turbulence_scale) */
     float flow_tcorr;       /* sheared turbulence: correlation time in orbital periods (flow.correlation_orbits) */
+    int sky_on;             /* visible mode: escaped rays take the checkerboard sky (sky_on, scene.hpp:58) */
+    int sky_nu, sky_nv;     /* squares of the checkerboard along galactic u (longitude) and v (latitude) */
+    float sky_scale;        /* radiance of the bright squares; dark ones are 0.1 of it (sky.brightness) */
+    double sky_rot[9];      /* black-hole frame -> galactic, row-major (sky_rot, scene.hpp:61) */
 } bh_scene;
 
 /* SceneTables (renderer.hpp:19-23) plus the turbulence input. The K2 table is ln K2(1/Theta) on a
@@ -121,6 +127,10 @@ typedef struct bh_trace_opts {
                          bh_render_adaptive alone; frames beyond 2^31 samples on the frame kernel.
                          BH_ENGINE_FRAME -> always the frame kernel. Both give the same pixels and
                          counters. */
+    double tol_t;     /* mixed: absolute bound (M) on one step's coordinate-time error, which the
+                         reference's error norm leaves out; it matters where the turbulence depends on
+                         the emission time (grid time slices, the sheared model). 0 -> the default:
+                         such scenes get a bound, others none; < 0 -> no bound */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */
@@ -144,6 +154,7 @@ typedef struct bh_ray_out {
     double trans;
     double psi;               /* swept angle (radio) */
     int fate, ncross, order, steps, accepted, flow_steps;
+    float sky_u, sky_v;       /* visible mode, escaped rays: galactic texture coordinates (uv_out, trace.cuh:62) */
 } bh_ray_out;
 
 typedef struct bh_ctx bh_ctx;
@@ -380,7 +391,7 @@ int bh_tonemap_hud(bh_ctx* ctx, const float* px, int W, int H, const bh_tone_par
 
 /* Diagnostic: evaluates one elementary function of the device math library on n inputs (no context
    needed; device 0). fn: 0 exp, 1 log, 2 expm1, 3 cbrt, 4 x^-0.2, 5 sin, 6 cos, 7 atan2(x, y),
-   8 acos, 9 sqrt, 10 x / y. Used by the tests that pin the device functions bit for bit. */
+   8 acos, 9 sqrt, 10 x / y, 11 asin. Used by the tests that pin the device functions bit for bit. */
 int bh_math_eval(int fn, const double* x, const double* y, double* out, int n);
 
 /* Renderer::~Renderer                                           render.cu:526-529 */

@@ -28,6 +28,7 @@
 #define M_EXPM1(x) bhm::expm1(x)
 #define M_ATAN2(y, x) bhm::atan2(y, x)
 #define M_ACOS(x) bhm::acos(x)
+#define M_ASIN(x) bhm::asin(x)
 static inline void M_SINCOS(double x, double* s, double* c) { bhm::sincos(x, s, c); }
 #else
 #define M_POWM02(x) std::pow(x, -0.2)
@@ -37,6 +38,7 @@ static inline void M_SINCOS(double x, double* s, double* c) { bhm::sincos(x, s, 
 #define M_EXPM1(x) std::expm1(x)
 #define M_ATAN2(y, x) std::atan2(y, x)
 #define M_ACOS(x) std::acos(x)
+#define M_ASIN(x) std::asin(x)
 static inline void M_SINCOS(double x, double* s, double* c) {
     *s = std::sin(x);
     *c = std::cos(x);
@@ -496,8 +498,32 @@ void or_escape_angles(const double* y, const double* k, double* th, double* ph) 
     *ph = y[4] + y[0] * dphdr;
 }
 
-// blackholes/src/trace.cuh:62-165 without the disk (106-128) and sky (154-160) branches; the
-// intensity accumulates in FP64 (the reference sums (float)(trans*dI) into a float4).
+// blackholes/src/trace.cuh:25-33
+void or_sky_uv(const double rot[9], double th, double ph, float* u, float* v) {
+    double st, ct, sp, cp;
+    M_SINCOS(th, &st, &ct);
+    M_SINCOS(ph, &sp, &cp);
+    const double d[3] = {st * cp, st * sp, ct};
+    double g[3];
+    for (int i = 0; i < 3; ++i) g[i] = rot[3 * i] * d[0] + rot[3 * i + 1] * d[1] + rot[3 * i + 2] * d[2];
+    const double l = M_ATAN2(g[1], g[0]), b = M_ASIN(std::fmax(-1.0, std::fmin(1.0, g[2])));
+    float uu = (float)(0.5 - l * (0.5 / PI));
+    uu -= std::floor(uu);
+    *u = uu;
+    *v = (float)(0.5 - b / PI);
+}
+
+// The procedural sky of this path (the reference samples a star-map texture, trace.cuh:35-45)
+float or_sky_checker(int nu, int nv, float scale, float u, float v) {
+    int iu = (int)(u * (float)nu), iv = (int)(v * (float)nv);
+    iu = std::min(std::max(iu, 0), nu - 1);
+    iv = std::min(std::max(iv, 0), nv - 1);
+    return ((iu + iv) & 1) ? scale : 0.1f * scale;
+}
+
+// blackholes/src/trace.cuh:62-165 without the disk branch (106-128); in visible mode an escaped ray
+// takes the procedural checkerboard sky at its galactic (u, v) (the reference's star-map texture,
+// 154-160); the intensity accumulates in FP64 (the reference sums floats into a float4).
 void or_trace_ray(const or_scene* S, const or_frame* F, double x, double yv, or_ray_out* o) {
     std::memset(o, 0, sizeof(*o));
     const double a = S->a;
@@ -573,6 +599,15 @@ void or_trace_ray(const or_scene* S, const or_frame* F, double x, double yv, or_
         if (y[0] > S->r_escape && y[1] > 0.0) {
             fate = 1;
             or_escape_angles(y, k1, &th_inf, &ph_inf);
+            if (!radio) {
+                or_sky_uv(S->sky_rot, th_inf, ph_inf, &o->sky_u, &o->sky_v);
+                if (S->sky_on) {
+                    const double s = (double)or_sky_checker(S->sky_nu, S->sky_nv, S->sky_scale, o->sky_u, o->sky_v);
+                    acc[0] += trans * s;
+                    acc[1] += trans * s;
+                    acc[2] += trans * s;
+                }
+            }
             break;
         }
     }
@@ -726,6 +761,7 @@ void or_math_eval(int fn, const double* x, const double* y, double* out, int n) 
             case 7: r = M_ATAN2(a, b); break;
             case 8: r = M_ACOS(a); break;
             case 9: r = std::sqrt(a); break;
+            case 11: r = M_ASIN(a); break;
             default: r = a / b; break;
         }
         out[i] = r;

@@ -52,6 +52,12 @@ typedef struct {
     int turb_model;
     const float* ts_slices;
     or_sheared ts;
+    // Visible mode: procedural checkerboard celestial sphere on the galactic (u, v) of sky_uv
+    // (sky_nu x sky_nv squares, bright sky_scale, dark 0.1 sky_scale); sky_rot: black-hole frame ->
+    // galactic, row-major (scene_build.cpp:181-204)
+    int sky_on, sky_nu, sky_nv;
+    float sky_scale;
+    double sky_rot[9];
 } or_scene;
 
 typedef struct {
@@ -75,6 +81,7 @@ typedef struct {
     int steps;            // DP54 step attempts
     int accepted;
     int flow_steps;       // accepted steps inside r < r_flow_max (radio)
+    float sky_u, sky_v;   // visible mode, escaped rays: galactic texture coordinates (trace.cuh:25-33)
 } or_ray_out;
 
 // kerr.hpp
@@ -117,6 +124,10 @@ double or_turb_factor(const or_scene* S, double r, double u, double phi, double 
 void or_flow_step(const or_scene* S, double r, double u, double phi, double t, double lam, double Ecam,
                   double h, double* dI, double* dtau);
 void or_escape_angles(const double* y, const double* k, double* th, double* ph);
+// Galactic texture coordinates of an escape direction (trace.cuh:25-33) through the row-major rotation rot
+void or_sky_uv(const double rot[9], double th, double ph, float* u, float* v);
+// Procedural checkerboard radiance at (u, v): nu x nv squares, bright `scale`, dark 0.1 scale
+float or_sky_checker(int nu, int nv, float scale, float u, float v);
 void or_trace_ray(const or_scene* S, const or_frame* F, double x, double yv, or_ray_out* out);
 
 // Frame drivers (std::thread, dynamic row scheduling). Rays: pixel list (px[i], py[i]), spp samples

@@ -33,6 +33,8 @@ class OrScene(C.Structure):
         ("k2_lntab", C.POINTER(C.c_float)), ("k2_n", C.c_int), ("k2_l0", C.c_float), ("k2_dl", C.c_float),
         ("seed", C.c_uint32), ("flow_turb_scale", C.c_float), ("flow_tcorr", C.c_float),
         ("turb_model", C.c_int), ("ts_slices", C.POINTER(C.c_float)), ("ts", OrSheared),
+        ("sky_on", C.c_int), ("sky_nu", C.c_int), ("sky_nv", C.c_int), ("sky_scale", C.c_float),
+        ("sky_rot", C.c_double * 9),
     ]
 
 
@@ -50,14 +52,14 @@ class OrRay(C.Structure):
         ("y", C.c_double * 6), ("th_inf", C.c_double), ("ph_inf", C.c_double), ("I", C.c_double * 4),
         ("trans", C.c_double), ("psi", C.c_double),
         ("fate", C.c_int), ("ncross", C.c_int), ("order", C.c_int), ("steps", C.c_int),
-        ("accepted", C.c_int), ("flow_steps", C.c_int),
+        ("accepted", C.c_int), ("flow_steps", C.c_int), ("sky_u", C.c_float), ("sky_v", C.c_float),
     ]
 
 
 RAY_DTYPE = np.dtype([
     ("y", "<f8", (6,)), ("th_inf", "<f8"), ("ph_inf", "<f8"), ("I", "<f8", (4,)), ("trans", "<f8"),
     ("psi", "<f8"), ("fate", "<i4"), ("ncross", "<i4"), ("order", "<i4"), ("steps", "<i4"),
-    ("accepted", "<i4"), ("flow_steps", "<i4"),
+    ("accepted", "<i4"), ("flow_steps", "<i4"), ("sky_u", "<f4"), ("sky_v", "<f4"),
 ])
 assert RAY_DTYPE.itemsize == C.sizeof(OrRay)
 
@@ -145,9 +147,26 @@ class Oracle:
                                       C.c_int, _D, C.c_void_p]
         L.or_selftest.argtypes = [_D, _D, _I]
         L.or_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
+        L.or_sky_uv.argtypes = [_D, C.c_double, C.c_double, _F, _F]
+        L.or_sky_checker.restype = C.c_float
+        L.or_sky_checker.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
 
     MATH_FN = {"exp": 0, "log": 1, "expm1": 2, "cbrt": 3, "pow_m02": 4, "sin": 5, "cos": 6, "atan2": 7,
-               "acos": 8, "sqrt": 9, "div": 10}
+               "acos": 8, "sqrt": 9, "div": 10, "asin": 11}
+
+    def sky_uv(self, rot, th, ph):
+        """Galactic (u, v) of escape directions (th, ph) through the row-major 3 x 3 rotation rot."""
+        r = np.ascontiguousarray(rot, dtype=np.float64).ravel()
+        th, ph = np.broadcast_arrays(np.asarray(th, dtype=np.float64), np.asarray(ph, dtype=np.float64))
+        u, v = np.zeros(th.shape, np.float32), np.zeros(th.shape, np.float32)
+        uu, vv = C.c_float(), C.c_float()
+        for i, (a, b) in enumerate(zip(th.ravel(), ph.ravel())):
+            self.lib.or_sky_uv(_dp(r), float(a), float(b), C.byref(uu), C.byref(vv))
+            u.flat[i], v.flat[i] = uu.value, vv.value
+        return u, v
+
+    def sky_checker(self, nu, nv, scale, u, v):
+        return self.lib.or_sky_checker(nu, nv, scale, float(u), float(v))
 
     def math_eval(self, fn, x, y=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -453,6 +472,13 @@ class Ref:
         if not ok:
             raise RuntimeError(f"reference build_scene failed for {path}")
         return out, k2
+
+    def sky_rot(self, path, sets=""):
+        """(sky_rot row-major 9 values, sky_scale) of the reference's build_scene."""
+        out = np.zeros(10)
+        if not self.lib.ref_sky_rot(path.encode(), sets.encode(), _dp(out)):
+            raise RuntimeError(f"reference build_scene failed for {path}")
+        return out[:9], out[9]
 
     def build_shot(self, path, sets="", maxn=4096):
         out = np.zeros((maxn, 20))

@@ -104,6 +104,25 @@ int ref_build_scene(const char* path, const char* sets, double* out, float* k2) 
     return 1;
 }
 
+// sky_rot of build_scene (scene_build.cpp:181-204), row-major, and sky_scale
+int ref_sky_rot(const char* path, const char* sets, double* out) {
+    Config c;
+    if (!load_cfg(path, sets, &c)) return 0;
+    SceneGPU S;
+    SceneTables T;
+    SceneInfo I;
+    std::string err;
+    if (!build_scene(c, &S, &T, &I, &err)) return 0;
+    const dvec3 rows[3] = {S.sky_rot.r0, S.sky_rot.r1, S.sky_rot.r2};
+    for (int i = 0; i < 3; ++i) {
+        out[3 * i] = rows[i].x;
+        out[3 * i + 1] = rows[i].y;
+        out[3 * i + 2] = rows[i].z;
+    }
+    out[9] = S.sky_scale;
+    return 1;
+}
+
 // Frames of a shot: per frame 14 values (t, r, incl, phi, roll, yaw, pitch, fov, fov_in_m, zoom,
 // target, off_x, off_y, time_m). Returns the frame count (at most maxn written).
 int ref_build_shot(const char* path, const char* sets, double* out, int maxn) {

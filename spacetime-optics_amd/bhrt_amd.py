@@ -43,6 +43,8 @@ class Scene(C.Structure):
         ("h_r", C.c_double), ("beta_p", C.c_double), ("r_flow_max", C.c_double), ("nu_obs", C.c_double),
         ("rg_cm", C.c_double), ("flow_kep", C.c_double), ("flow_turb_amp", C.c_float), ("seed", C.c_uint32),
         ("flow_turb_scale", C.c_float), ("flow_tcorr", C.c_float),
+        ("sky_on", C.c_int), ("sky_nu", C.c_int), ("sky_nv", C.c_int), ("sky_scale", C.c_float),
+        ("sky_rot", C.c_double * 9),
     ]
 
 
@@ -71,7 +73,7 @@ class Settings(C.Structure):
 class TraceOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
                 ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int),
-                ("full_flow", C.c_int), ("engine", C.c_int)]
+                ("full_flow", C.c_int), ("engine", C.c_int), ("tol_t", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -153,7 +155,7 @@ class MlpBakeReport(C.Structure):
 RAY_DTYPE = np.dtype([
     ("y", "<f8", (6,)), ("th_inf", "<f8"), ("ph_inf", "<f8"), ("I", "<f8", (4,)), ("trans", "<f8"),
     ("psi", "<f8"), ("fate", "<i4"), ("ncross", "<i4"), ("order", "<i4"), ("steps", "<i4"),
-    ("accepted", "<i4"), ("flow_steps", "<i4"),
+    ("accepted", "<i4"), ("flow_steps", "<i4"), ("sky_u", "<f4"), ("sky_v", "<f4"),
 ])
 
 _D = C.POINTER(C.c_double)
@@ -383,7 +385,7 @@ def k2_from_scene(ini, sets=""):
 
 
 MATH_FN = {"exp": 0, "log": 1, "expm1": 2, "cbrt": 3, "pow_m02": 4, "sin": 5, "cos": 6, "atan2": 7, "acos": 8,
-           "sqrt": 9, "div": 10}
+           "sqrt": 9, "div": 10, "asin": 11}
 
 
 def math_eval(fn, x, y=None):
@@ -520,8 +522,9 @@ class Tracer:
 
     @staticmethod
     def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0, r_emit=0.0,
-             far_fp32=0, full_flow=0, engine=ENGINE_AUTO):
-        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow, engine)
+             far_fp32=0, full_flow=0, engine=ENGINE_AUTO, tol_t=0.0):
+        return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow, engine,
+                         tol_t)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)

@@ -19,7 +19,7 @@
    - sincos: Cody-Waite reduction by pi/2 with a three-part constant (33+33+53 bits), Taylor series
      on |r| <= pi/4 (sin to r^17, cos to r^18).
    - atan2/acos: reciprocal and pi/6 (Hart) reductions to |t| <= tan(pi/12), odd Taylor series to
-     t^29; acos(x) = atan2(sqrt((1-x)(1+x)), x).
+     t^29; acos(x) = atan2(sqrt((1-x)(1+x)), x), asin(x) = atan2(x, sqrt((1-x)(1+x))).
    - x^-0.2 (the step controller): exp(-0.2 log x) refined by one Newton step on x y^5 = 1 with an
      FMA double-double residual; correctly rounded on [1e-300, 1e300]. FMA is exact on both sides
      (v_fma_f64 on the device, the C library's fma on the host), so this stays bit-identical.
@@ -287,6 +287,11 @@ BHM_HD double atan2(double y, double x) {
 BHM_HD double acos(double x) {
     if (isnan_(x) || x > 1.0 || x < -1.0) return __builtin_nan("");
     return atan2(__builtin_sqrt((1.0 - x) * (1.0 + x)), x);
+}
+
+BHM_HD double asin(double x) {
+    if (isnan_(x) || x > 1.0 || x < -1.0) return __builtin_nan("");
+    return atan2(x, __builtin_sqrt((1.0 - x) * (1.0 + x)));
 }
 
 }  // namespace bhm

@@ -104,6 +104,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, const bhm3
                 q->steps = r.steps;
                 q->accepted = r.accepted;
                 q->flow_steps = r.flow_steps;
+                q->sky_u = r.sky_u;
+                q->sky_v = r.sky_v;
             } else {
                 const double m = (double)P.spp;   // mean as sum / count, as the adaptive resolve and the oracle
                 px_out[o] = make_float4((float)(a0 / m), (float)(a1 / m), (float)(a2 / m), (float)(a3 / m));
@@ -486,6 +488,7 @@ __global__ void k_math_eval(int fn, const double* __restrict__ x, const double* 
         case 6: bhm::sincos(a, &s, &c); r = c; break;
         case 7: r = bhm::atan2(a, b); break;
         case 8: r = bhm::acos(a); break;
+        case 11: r = bhm::asin(a); break;
         case 9: r = sqrt(a); break;
         default: r = a / b; break;
     }
@@ -577,6 +580,11 @@ TraceParams make_params_scene(const bh_ctx* c, const bh_scene& S, const bh_frame
         P.turb_on = (S.flow_turb_amp > 0.f && c->ts_valid) ? 1 : 0;
         P.ts = c->d_tsv;
     }
+    P.sky_on = S.mode == BH_MODE_VISIBLE && S.sky_on && S.sky_nu > 0 && S.sky_nv > 0 ? 1 : 0;
+    P.sky_nu = S.sky_nu;
+    P.sky_nv = S.sky_nv;
+    P.sky_scale = S.sky_scale;
+    for (int i = 0; i < 9; ++i) P.sky_rot[i] = S.sky_rot[i];
     if (f) {
         P.ph0 = f->ph;
         P.t0 = f->t;
@@ -743,6 +751,9 @@ double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2
     return r0;
 }
 
+// Default bound on a mixed-tracer step's coordinate-time error (M) for time-dependent turbulence
+constexpr double DEFAULT_TOL_T = 1e-3;
+
 // Log-space emission constants and the region split of the mixed policy
 bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, const bh_trace_opts& o) {
     constexpr double C = C_C, H = C_H, ME = C_ME, MP = C_MP, QE = C_QE, PI = C_PI;
@@ -788,6 +799,12 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
         if (!o.full_flow) M.tol_mid = std::max(M.tol_mid, M.tol_far * 0.1);
     }
     M.r_f32 = o.far_fp32 ? M.r_far : 1e300;
+    // coordinate-time bound: only where the emission depends on the time a sample is emitted at
+    const bool time_dependent = S.mode == BH_MODE_RADIO && S.flow_on && S.flow_turb_amp > 0.f &&
+                                ((c->turb_model == BH_TURB_GRID && c->d_turb && c->turb_nt > 1) ||
+                                 c->turb_model == BH_TURB_SHEARED);
+    const double tol_t = o.tol_t > 0 ? o.tol_t : (o.tol_t == 0 && time_dependent ? DEFAULT_TOL_T : 0.0);
+    M.inv_tol_t = tol_t > 0 ? 1.0 / tol_t : 0.0;
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);
     E.n_index = (float)S.n_index;
@@ -1707,7 +1724,7 @@ int bh_vacuum_fates(bh_ctx* c, const double* photons, int n, double tol, double 
 }
 
 int bh_math_eval(int fn, const double* x, const double* y, double* out, int n) {
-    if (!x || !y || !out || n < 0 || fn < 0 || fn > 10) return BH_ERR_ARG;
+    if (!x || !y || !out || n < 0 || fn < 0 || fn > 11) return BH_ERR_ARG;
     if (n == 0) return BH_OK;
     if (bh_device_count() <= 0) return BH_ERR_NODEVICE;
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;

@@ -57,8 +57,9 @@ bool write_frame(const std::string& path, const bh::HdrFrame& f) {
 
 // Phase 2 of the reference's frame loop (main.cpp:569-603): exposure from the smoothed display
 // statistics of all frames, radio tone map, PNG per frame (and the HDR frame with output.hdr = 1).
-// Visible-mode frames (vacuum geodesics: the disk and sky shading are not part of this tracer) are
-// written as HDR only.
+// Visible-mode frames (vacuum geodesics and the checkerboard sky; the disk and the reference's ACES
+// display map are not part of this tracer) are written as a linear grey PNG of the middle channel
+// times post.exposure (default 1), and as HDR with output.hdr = 1.
 // What the HUD layout needs besides the configuration and the shot
 struct HudInputs {
     const bh::Font* font = nullptr;   // null: no overlay
@@ -71,10 +72,24 @@ bool tone_and_write(const bh::Config& cfg, const bh::Shot& shot, bh::Renderer& R
                     const HudInputs& hud) {
     const size_t n = frames.size();
     if (!radio) {
+        const double gain = cfg.str("post.exposure", "auto") == "auto" ? 1.0 : cfg.num("post.exposure", 1.0);
+        const bool hdr = cfg.flag("output.hdr", false);
+        std::vector<uint8_t> grey;
         for (size_t i = 0; i < n; ++i) {
+            const bh::HdrFrame& fr = frames[i];
+            grey.resize((size_t)fr.W * fr.H);
+            for (size_t p = 0; p < grey.size(); ++p) {
+                const double v = std::min(1.0, std::max(0.0, gain * fr.px[4 * p + 1]));
+                grey[p] = (uint8_t)std::lround(255.0 * v);
+            }
             char name[64];
+            std::snprintf(name, sizeof name, "/frames/frame_%05d.png", index[i]);
+            if (!bh::write_png(out + name, fr.W, fr.H, 1, grey.data())) {
+                std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
+                return false;
+            }
             std::snprintf(name, sizeof name, "/frames/frame_%05d.bhf", index[i]);
-            if (!write_frame(out + name, frames[i])) {
+            if (hdr && !write_frame(out + name, frames[i])) {
                 std::fprintf(stderr, "bhrt: cannot write %s%s\n", out.c_str(), name);
                 return false;
             }

@@ -1,6 +1,7 @@
 #include "scene.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <sstream>
@@ -102,6 +103,37 @@ void smooth_axis(std::vector<double>& f, size_t count, size_t len, size_t stride
     for (auto& t : th) t.join();
 }
 
+// Black-hole frame -> galactic rotation of the celestial sphere (scene_build.cpp:181-204). Two
+// orthonormal bases are matched: the camera's (the direction straight behind the hole, the sky's up
+// seen from there, and their cross product) and the galactic one around the direction (l, b) (that
+// direction, its north turned by roll, and their cross product); M = G B^T maps the first onto the
+// second. Same operation order as the reference (its shot fixtures pin it).
+void sky_rotation(double incl, double phi, double l, double b, double roll, double M[9]) {
+    using A3 = std::array<double, 3>;
+    auto dot3 = [](const A3& p, const A3& q) { return p[0] * q[0] + p[1] * q[1] + p[2] * q[2]; };
+    auto minus_along = [](const A3& p, const A3& v, double k) { return A3{p[0] - v[0] * k, p[1] - v[1] * k, p[2] - v[2] * k}; };
+    auto unit = [&](const A3& v) {
+        const double k = 1.0 / std::sqrt(dot3(v, v));
+        return A3{v[0] * k, v[1] * k, v[2] * k};
+    };
+    auto cross3 = [](const A3& p, const A3& q) {
+        return A3{p[1] * q[2] - p[2] * q[1], p[2] * q[0] - p[0] * q[2], p[0] * q[1] - p[1] * q[0]};
+    };
+    const A3 pole = {0.0, 0.0, 1.0};
+    const A3 behind = {-(std::sin(incl) * std::cos(phi)), -(std::sin(incl) * std::sin(phi)), -std::cos(incl)};
+    A3 up = unit(minus_along(pole, behind, dot3(pole, behind)));
+    if (!(std::sqrt(dot3(up, up)) > 0.5)) up = {1.0, 0.0, 0.0};   // camera on the pole
+    const A3 side = cross3(up, behind);
+    const A3 gt = {std::cos(b) * std::cos(l), std::cos(b) * std::sin(l), std::sin(b)};
+    const A3 gn = unit(minus_along(pole, gt, dot3(pole, gt)));
+    const A3 gs = cross3(gn, gt);
+    const double cr = std::cos(roll), sr = std::sin(roll);
+    const A3 gu = {gn[0] * cr + gs[0] * sr, gn[1] * cr + gs[1] * sr, gn[2] * cr + gs[2] * sr};
+    const A3 gv = cross3(gu, gt);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[3 * i + j] = gt[i] * behind[j] + gu[i] * up[j] + gv[i] * side[j];
+}
+
 }  // namespace
 
 void make_turbulence(int n, int nt, double half, double sigma, double corr_m, double corr_t, double dt,
@@ -180,6 +212,27 @@ bool build_scene(const Config& c, bh_scene* S, SceneTables* T, SceneInfo* info, 
     S->flow_turb_amp = (float)c.num("flow.turbulence", 0.7);
     S->flow_turb_scale = (float)c.num("flow.turbulence_scale", 0.45);
     S->flow_tcorr = (float)c.num("flow.correlation_orbits", 0.5);
+    // Celestial sphere (visible mode): the reference's [sky] orientation keys (scene_build.cpp:181-204)
+    // and brightness; a procedural checkerboard of sky.checker = nu, nv squares (default 24, 12:
+    // 15 degree squares; 0 turns the sky off) in place of its star-map file
+    {
+        auto rad = [&](const char* key, double dflt) { return c.num(key, dflt) * PI / 180; };   // (deg * pi) / 180
+        sky_rotation(rad("camera.incl", 80.0), rad("camera.phi", 0.0), rad("sky.l_deg", 0.0), rad("sky.b_deg", 0.0),
+                     rad("sky.roll_deg", 0.0), S->sky_rot);
+        S->sky_scale = (float)c.num("sky.brightness", 1.0);
+        std::stringstream ss(c.str("sky.checker", "24, 12"));
+        std::string item;
+        int n[2] = {0, 0};
+        for (int i = 0; i < 2 && std::getline(ss, item, ','); ++i) n[i] = std::atoi(item.c_str());
+        if (n[1] <= 0) n[1] = n[0] / 2;
+        if (n[0] < 0 || n[1] < 0 || n[0] > 4096 || n[1] > 4096) {
+            if (err) *err = "sky.checker must be two counts in [0, 4096]";
+            return false;
+        }
+        S->sky_nu = n[0];
+        S->sky_nv = n[1];
+        S->sky_on = S->mode == BH_MODE_VISIBLE && n[0] > 0 && n[1] > 0 ? 1 : 0;
+    }
     info->radio_flux_target_jy = c.num("flow.flux_jy", 0.0);
     info->normalize_fov_m = c.num("flow.normalize_fov_m", 40.0);
     k2_table(512, &T->k2_ln, &T->k2_l0, &T->k2_dl);

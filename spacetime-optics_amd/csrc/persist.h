@@ -129,8 +129,8 @@ struct Tracer<BH_PREC_FP64, RADIO> {
     static __device__ __forceinline__ void step(const TraceParams& P, const bhm32::MixedParams&, State& R) {
         bhd::ray_step<RADIO>(P, R);
     }
-    static __device__ __forceinline__ int result(const State& R, RayResult& out) {
-        bhd::ray_result(R, out);
+    static __device__ __forceinline__ int result(const TraceParams& P, const State& R, RayResult& out) {
+        bhd::ray_result(P, R, out);
         return out.steps;
     }
 };
@@ -145,8 +145,8 @@ struct Tracer<BH_PREC_MIXED, RADIO> {
     static __device__ __forceinline__ void step(const TraceParams& P, const bhm32::MixedParams& M, State& R) {
         bhm32::ray_step_m<RADIO>(P, M, R);
     }
-    static __device__ __forceinline__ int result(const State& R, RayResult& out) {
-        return bhm32::ray_result_m(R, out);
+    static __device__ __forceinline__ int result(const TraceParams& P, const State& R, RayResult& out) {
+        return bhm32::ray_result_m(P, R, out);
     }
 };
 
@@ -222,7 +222,7 @@ __device__ __forceinline__ void persist_body(const Work& Wk, unsigned int* __res
                 if (!R.done) T::step(P, M, R);
                 if (R.done) {
                     RayResult r;
-                    const int f64 = T::result(R, r);
+                    const int f64 = T::result(P, R, r);
                     Wk.finish(cf, ent, r);
                     s_flux[tid] += r.I[0];
                     s_stat[0][tid] += (unsigned long long)r.steps;

@@ -29,6 +29,7 @@ constexpr double PI = 3.14159265358979323846;
 // turbulence or with the sheared turbulence. The turbulence model is a compile-time choice so a
 // kernel carries the lookup code of one model only (registers and instruction cache).
 constexpr int RADIO_OFF = 0, RADIO_GRID = 1, RADIO_SHEARED = 2;
+constexpr int MODE_VISIBLE = 0;   // bh_scene.mode: vacuum geodesics and the celestial sphere
 
 // Dormand-Prince 1980 tableau (kerr.hpp:124-129): the same constant expressions, folded by the compiler.
 constexpr double a21 = 1.0 / 5, a31 = 3.0 / 40, a32 = 9.0 / 40, a41 = 44.0 / 45, a42 = -56.0 / 15, a43 = 32.0 / 9;
@@ -69,6 +70,10 @@ struct TraceParams {
     double cx, cy, pix;
     int W, H;
     uint32_t seed;
+    // visible mode: checkerboard celestial sphere (sky_nu x sky_nv squares of galactic u, v)
+    int sky_on, sky_nu, sky_nv;
+    float sky_scale;
+    double sky_rot[9];
     // launch
     int x0, y0, w, h, spp, jitter;
 };
@@ -79,6 +84,7 @@ struct RayResult {
     double I[4];
     double trans, psi;
     int fate, ncross, order, steps, accepted, flow_steps;
+    float sky_u, sky_v;
 };
 
 struct RayConst {
@@ -485,7 +491,47 @@ __device__ __forceinline__ void ray_step(const TraceParams& P, RayState& R) {
     if (R.it >= P.max_steps) R.done = true;
 }
 
-__device__ __forceinline__ void ray_result(const RayState& R, RayResult& out) {
+// blackholes/src/trace.cuh:25-33: galactic texture coordinates of the escape direction (th, ph),
+// the direction rotated by sky_rot (black-hole frame -> galactic) and taken to (longitude, latitude)
+__device__ __forceinline__ void sky_uv(const TraceParams& P, double th, double ph, float* u, float* v) {
+    double st, ct, sp, cp;
+    bhm::sincos(th, &st, &ct);
+    bhm::sincos(ph, &sp, &cp);
+    const double dx = st * cp, dy = st * sp, dz = ct;
+    const double gx = P.sky_rot[0] * dx + P.sky_rot[1] * dy + P.sky_rot[2] * dz;
+    const double gy = P.sky_rot[3] * dx + P.sky_rot[4] * dy + P.sky_rot[5] * dz;
+    const double gz = P.sky_rot[6] * dx + P.sky_rot[7] * dy + P.sky_rot[8] * dz;
+    const double l = bhm::atan2(gy, gx), b = bhm::asin(fmax(-1.0, fmin(1.0, gz)));
+    float uu = (float)(0.5 - l * (0.5 / PI));
+    uu -= floorf(uu);
+    *u = uu;
+    *v = (float)(0.5 - b / PI);
+}
+
+// Radiance of the procedural sky at (u, v): sky_nu x sky_nv squares, bright sky_scale, dark 0.1 of it
+// (the reference samples its star-map texture here, trace.cuh:35-45)
+__device__ __forceinline__ float sky_checker(const TraceParams& P, float u, float v) {
+    const int iu = min(max((int)(u * (float)P.sky_nu), 0), P.sky_nu - 1);
+    const int iv = min(max((int)(v * (float)P.sky_nv), 0), P.sky_nv - 1);
+    return ((iu + iv) & 1) ? P.sky_scale : 0.1f * P.sky_scale;
+}
+
+// Visible mode, escaped ray (trace.cuh:154-160): sky coordinates and, with the sky on, its radiance
+// in the first three channels (the reference's X, Y, Z), weighted by the transmission
+__device__ __forceinline__ void sky_result(const TraceParams& P, double trans, RayResult& out) {
+    out.sky_u = 0.f;
+    out.sky_v = 0.f;
+    if (P.mode != MODE_VISIBLE || out.fate != 1) return;
+    sky_uv(P, out.th_inf, out.ph_inf, &out.sky_u, &out.sky_v);
+    if (P.sky_on) {
+        const double s = (double)sky_checker(P, out.sky_u, out.sky_v);
+        out.I[0] += trans * s;
+        out.I[1] += trans * s;
+        out.I[2] += trans * s;
+    }
+}
+
+__device__ __forceinline__ void ray_result(const TraceParams& P, const RayState& R, RayResult& out) {
     out.y[0] = R.s.y0; out.y[1] = R.s.y1; out.y[2] = R.s.y2; out.y[3] = R.s.y3; out.y[4] = R.s.y4;
     out.y[5] = R.s.y5;
     out.th_inf = 0.0;
@@ -507,15 +553,16 @@ __device__ __forceinline__ void ray_result(const RayState& R, RayResult& out) {
     out.steps = R.it;
     out.accepted = R.accepted;
     out.flow_steps = R.flow_steps;
+    sky_result(P, R.trans, out);
 }
 
-// blackholes/src/trace.cuh:62-165 (disk and sky branches out of scope), FP64 intensity.
+// blackholes/src/trace.cuh:62-165 (disk branch out of scope, procedural sky), FP64 intensity.
 template <int RADIO>
 __device__ __forceinline__ void trace_ray_fp64(const TraceParams& P, double x, double yv, RayResult& out) {
     RayState R;
     ray_init<RADIO>(P, x, yv, R);
     while (!R.done) ray_step<RADIO>(P, R);
-    ray_result(R, out);
+    ray_result(P, R, out);
 }
 
 // blackholes/src/kerr.hpp:184-203 for one photon (lam, eta, E, y[6])

@@ -124,6 +124,7 @@ __device__ __forceinline__ Rhs<R> rhs(const RConst<R>& c, R r, R u) {
    the 5th-order state accumulated in FP64 and the scaled error norm in FP64. */
 template <typename R>
 __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, double h, double tol, double tol4,
+                                       double inv_tol_t,
                                        State& o) {
     using namespace bhd;
     const R hr = (R)h;
@@ -135,7 +136,8 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     R S6_0 = R(a61) * q0, S6_1 = R(a61) * q1, S6_2 = R(a61) * q2, S6_3 = R(a61) * q3;
     R O0 = R(b1) * q0, O1 = R(b1) * q1, O2 = R(b1) * q2, O3 = R(b1) * q3, O4 = R(b1) * (R)s.k4,
       O5 = R(b1) * (R)s.k5;
-    R E0 = R(e1) * q0, E1 = R(e1) * q1, E2 = R(e1) * q2, E3 = R(e1) * q3, E4 = R(e1) * (R)s.k4;
+    R E0 = R(e1) * q0, E1 = R(e1) * q1, E2 = R(e1) * q2, E3 = R(e1) * q3, E4 = R(e1) * (R)s.k4,
+      E5 = R(e1) * (R)s.k5;
     const R ha = hr * R(a21);
     R t0 = y0 + ha * q0, t1 = y1 + ha * q1, t2 = y2 + ha * q2, t3 = y3 + ha * q3;
     Rhs<R> k = rhs<R>(c, t0, t2);
@@ -151,25 +153,25 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     S5_0 += R(a53) * q0; S5_1 += R(a53) * q1; S5_2 += R(a53) * q2; S5_3 += R(a53) * q3;
     S6_0 += R(a63) * q0; S6_1 += R(a63) * q1; S6_2 += R(a63) * q2; S6_3 += R(a63) * q3;
     O0 += R(b3) * q0; O1 += R(b3) * q1; O2 += R(b3) * q2; O3 += R(b3) * q3; O4 += R(b3) * k.d4; O5 += R(b3) * k.d5;
-    E0 += R(e3) * q0; E1 += R(e3) * q1; E2 += R(e3) * q2; E3 += R(e3) * q3; E4 += R(e3) * k.d4;
+    E0 += R(e3) * q0; E1 += R(e3) * q1; E2 += R(e3) * q2; E3 += R(e3) * q3; E4 += R(e3) * k.d4; E5 += R(e3) * k.d5;
     t0 = y0 + hr * S4_0; t1 = y1 + hr * S4_1; t2 = y2 + hr * S4_2; t3 = y3 + hr * S4_3;
     k = rhs<R>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S5_0 += R(a54) * q0; S5_1 += R(a54) * q1; S5_2 += R(a54) * q2; S5_3 += R(a54) * q3;
     S6_0 += R(a64) * q0; S6_1 += R(a64) * q1; S6_2 += R(a64) * q2; S6_3 += R(a64) * q3;
     O0 += R(b4) * q0; O1 += R(b4) * q1; O2 += R(b4) * q2; O3 += R(b4) * q3; O4 += R(b4) * k.d4; O5 += R(b4) * k.d5;
-    E0 += R(e4) * q0; E1 += R(e4) * q1; E2 += R(e4) * q2; E3 += R(e4) * q3; E4 += R(e4) * k.d4;
+    E0 += R(e4) * q0; E1 += R(e4) * q1; E2 += R(e4) * q2; E3 += R(e4) * q3; E4 += R(e4) * k.d4; E5 += R(e4) * k.d5;
     t0 = y0 + hr * S5_0; t1 = y1 + hr * S5_1; t2 = y2 + hr * S5_2; t3 = y3 + hr * S5_3;
     k = rhs<R>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S6_0 += R(a65) * q0; S6_1 += R(a65) * q1; S6_2 += R(a65) * q2; S6_3 += R(a65) * q3;
     O0 += R(b5) * q0; O1 += R(b5) * q1; O2 += R(b5) * q2; O3 += R(b5) * q3; O4 += R(b5) * k.d4; O5 += R(b5) * k.d5;
-    E0 += R(e5) * q0; E1 += R(e5) * q1; E2 += R(e5) * q2; E3 += R(e5) * q3; E4 += R(e5) * k.d4;
+    E0 += R(e5) * q0; E1 += R(e5) * q1; E2 += R(e5) * q2; E3 += R(e5) * q3; E4 += R(e5) * k.d4; E5 += R(e5) * k.d5;
     t0 = y0 + hr * S6_0; t1 = y1 + hr * S6_1; t2 = y2 + hr * S6_2; t3 = y3 + hr * S6_3;
     k = rhs<R>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     O0 += R(b6) * q0; O1 += R(b6) * q1; O2 += R(b6) * q2; O3 += R(b6) * q3; O4 += R(b6) * k.d4; O5 += R(b6) * k.d5;
-    E0 += R(e6) * q0; E1 += R(e6) * q1; E2 += R(e6) * q2; E3 += R(e6) * q3; E4 += R(e6) * k.d4;
+    E0 += R(e6) * q0; E1 += R(e6) * q1; E2 += R(e6) * q2; E3 += R(e6) * q3; E4 += R(e6) * k.d4; E5 += R(e6) * k.d5;
     o.y0 = s.y0 + (double)(hr * O0);
     o.y1 = s.y1 + (double)(hr * O1);
     o.y2 = s.y2 + (double)(hr * O2);
@@ -182,6 +184,7 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     o.k4 = (double)k.d4;
     o.k5 = (double)k.d5;
     E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
+    E5 += R(e7) * k.d5;
     // scaled max-norm over r, r', u, u', phi (kerr.hpp:145-155) in FP64, with Newton-refined FP64
     // reciprocals of the scales in place of the divisions
     const double sc0 = tol * (1e-6 + fmax(fabs(s.y0), fabs(o.y0)));
@@ -192,6 +195,9 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     m = fmax(m, fabs(h * (double)E2) * rcpd(tol));
     m = fmax(m, fabs(h * (double)E3) * rcpd(sc3));
     m = fmax(m, fabs(h * (double)E4) * rcpd(tol4));
+    // coordinate time (not in the reference's norm): an absolute bound per step where the turbulence
+    // depends on the emission time (inv_tol_t = 1 / bound, 0 when it does not)
+    m = fmax(m, fabs(h * (double)E5) * inv_tol_t);
     return m;
 }
 
@@ -429,6 +435,7 @@ struct MixedParams {
     double r_cut;     // emission, absorption and the transfer step cap only below r_cut
     double tol_mid;   // tolerance between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
+    double inv_tol_t; // 1 / (absolute bound on a step's coordinate-time error); 0: no bound
     EmisParams E;
     int use_surrogate;
     float ln_e2_over_c, ln_e2_over_mec;   // ln(e^2 / c), ln(e^2 / (m_e c))
@@ -481,7 +488,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     double err;
     if (R.s.y0 >= M.r_f32) {
         const RConst<float> cf = make_const<float>(P, R.c);
-        err = dp54<float>(cf, R.s, hc, M.tol_far, M.tol_far * 4.0, n);
+        err = dp54<float>(cf, R.s, hc, M.tol_far, M.tol_far * 4.0, M.inv_tol_t, n);
     } else {
         const RConst<double> cd = make_const<double>(P, R.c);
         if (R.k1_f32) {   // entering the FP64 region: FP64 derivative at the current state
@@ -489,7 +496,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
         }
         const double tol = R.s.y0 < M.r_tol ? P.tol : (R.s.y0 < M.r_far ? M.tol_mid : M.tol_far);
-        err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, n);
+        err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, M.inv_tol_t, n);
         ++R.fp64_attempts;
     }
     R.k1_f32 = R.s.y0 >= M.r_f32;
@@ -570,7 +577,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
 }
 
 // Outputs of a finished ray; returns the number of FP64 step attempts.
-__device__ __forceinline__ int ray_result_m(const RayStateM& R, RayResult& out) {
+__device__ __forceinline__ int ray_result_m(const TraceParams& P, const RayStateM& R, RayResult& out) {
     out.y[0] = R.s.y0; out.y[1] = R.s.y1; out.y[2] = R.s.y2; out.y[3] = R.s.y3; out.y[4] = R.s.y4;
     out.y[5] = R.s.y5;
     out.th_inf = 0.0;
@@ -593,6 +600,7 @@ __device__ __forceinline__ int ray_result_m(const RayStateM& R, RayResult& out) 
     out.steps = R.it;
     out.accepted = R.accepted;
     out.flow_steps = R.flow_steps;
+    bhd::sky_result(P, R.trans, out);
     return R.fp64_attempts;
 }
 
@@ -604,7 +612,7 @@ __device__ __forceinline__ int trace_ray_mixed(const TraceParams& P, const Mixed
     RayStateM R;
     ray_init_m<RADIO>(P, M, x, yv, R);
     while (!R.done) ray_step_m<RADIO>(P, M, R);
-    return ray_result_m(R, out);
+    return ray_result_m(P, R, out);
 }
 
 #pragma clang fp contract(off)

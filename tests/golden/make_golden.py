@@ -7,7 +7,8 @@ critical_curve_point (blackholes/src/kerr.hpp), make_frame / find_critical_point
 (blackholes/src/shot.cpp), build_scene + K2 table (blackholes/src/scene_build.cpp) and Config
 (common/config.cpp). The inputs are seeded and stored beside the outputs.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        (ref_vectors.npz)
+    python tests/golden/make_golden.py sky    (sky_vectors.npz: sky_rot of build_scene)
 """
 import math
 import os
@@ -191,5 +192,36 @@ def main():
     print("wrote", os.path.join(HERE, "ref_vectors.npz"), {k: np.shape(v) for k, v in out.items()})
 
 
+def sky():
+    """tests/golden/sky_vectors.npz: sky_rot of the reference's build_scene for camera and [sky]
+    orientations (seeded, plus the pole and the reference's own sky shot)."""
+    ref = Ref()
+    rng = np.random.default_rng(20261203)
+    base = os.path.join(SHOTS, "c1_vacuum_256.ini")
+    rows = [(0.0, 0.0, 0.0, 0.0, 0.0), (180.0, 30.0, 10.0, -5.0, 20.0), (90.0, 0.0, 0.0, 0.0, 0.0),
+            (17.0, 0.0, 2.0, -2.0, 0.0)]
+    for _ in range(60):
+        rows.append((rng.uniform(0, 180), rng.uniform(-180, 360), rng.uniform(-180, 360), rng.uniform(-85, 85),
+                     rng.uniform(-180, 180)))
+    sets, rot, scale = [], [], []
+    for k, (incl, phi, l, b, roll) in enumerate(rows):
+        st = (f"camera.incl={incl!r};camera.phi={phi!r};sky.l_deg={l!r};sky.b_deg={b!r};sky.roll_deg={roll!r};"
+              f"sky.brightness={0.5 + k / 16!r}")
+        m, sc = ref.sky_rot(base, st)
+        sets.append(st)
+        rot.append(m)
+        scale.append(sc)
+    out = {"sky_sets": np.array(sets), "sky_rot": np.array(rot), "sky_scale": np.array(scale)}
+    gaia = os.path.join(REF_ROOT, "blackholes", "shots", "03_gaia_bh3.ini")
+    if os.path.exists(gaia):
+        m, sc = ref.sky_rot(gaia)
+        out["gaia_rot"] = m
+    np.savez_compressed(os.path.join(HERE, "sky_vectors.npz"), **out)
+    print("wrote", os.path.join(HERE, "sky_vectors.npz"), {k: np.shape(v) for k, v in out.items()})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["sky"]:
+        sky()
+    else:
+        main()

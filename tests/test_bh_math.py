@@ -5,10 +5,10 @@ The GPU tracer and the oracle's bh_math build share these functions, so the GPU/
 cannot see a wrong coefficient in them; this file can. Each function is evaluated by both oracle
 builds (liboracle_bhm: bh_math.h; liboracle_libm: glibc, the reference host build's arithmetic) on
 10^6 inputs spanning its domain plus special values, and the error is measured in units of the last
-place of the glibc result. glibc's exp/log/sin/cos/atan2/acos/cbrt/expm1/pow are correctly rounded
+place of the glibc result. glibc's exp/log/sin/cos/atan2/acos/asin/cbrt/expm1/pow are correctly rounded
 or within one ulp, so a bound of k ulp here means at most k + 1 ulp from the true value.
 
-Bounds asserted: exp, log <= 1 ulp; expm1, sin, cos <= 2; cbrt, atan2, acos <= 3; x^-0.2 (computed
+Bounds asserted: exp, log <= 1 ulp; expm1, sin, cos <= 2; cbrt, atan2, acos, asin <= 3; x^-0.2 (computed
 as exp(-0.2 log x), the step controller's factor, kerr.hpp/trace.cuh:93-94,146) <= 9, with mean
 error below 2 ulp. A dropped series term, a wrong reduction constant or a wrong quadrant fails
 these by many orders of magnitude."""
@@ -47,6 +47,7 @@ def _inputs(rng):
         "cos": (trig_x, None, 2.0),
         "atan2": (at_y, at_x, 3.0),
         "acos": (acos_x, None, 3.0),
+        "asin": (acos_x, None, 3.0),
     }
 
 
@@ -55,7 +56,7 @@ def inputs():
     return _inputs(np.random.default_rng(20240601))
 
 
-@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos"])
+@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos", "asin"])
 def test_ulp_bound_against_glibc(fn, inputs, oracle_bhm, oracle_libm):
     x, y, bound = inputs[fn]
     a = oracle_bhm.math_eval(fn, x, y)

@@ -53,11 +53,31 @@ def test_adaptive_equals_plain_mean_over_same_samples(prec):
 
 
 def test_adaptive_vacuum_takes_no_extra_samples():
-    """Visible mode drives the decision with channel 1, which the vacuum tracer leaves at zero."""
-    cfg = setup(shot("c1_vacuum_256.ini"), W=64, H=64)
+    """Visible mode drives the decision with channel 1, which the vacuum tracer leaves at zero when
+    the sky is off."""
+    cfg = setup(shot("c1_vacuum_256.ini"), "sky.checker=0", W=64, H=64)
     t = tracer(cfg, 2, 32, 0.03)
     img, ns, st = t.render_adaptive(cfg["F"], t.opts(jitter=1, spp=2))
     assert (ns == 2).all() and st.samples == 2 * 64 * 64
     plain, _ = t.render(cfg["F"], t.opts(jitter=1, spp=2))
-    assert np.array_equal(img, plain)
+    assert np.array_equal(img, plain) and not img.any()
+    t.close()
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_adaptive_sky_refines_edges(prec):
+    """With the checkerboard sky the samples of a pixel differ only where it straddles a square's edge
+    or the shadow's: those pixels (and their 3 x 3 neighbours) are refined, the rest keep 2 samples,
+    and every pixel equals the plain render at its sample count."""
+    cfg = setup(shot("c1_vacuum_256.ini"), W=64, H=64)
+    t = tracer(cfg, 2, 32, 0.03)
+    img, ns, st = t.render_adaptive(cfg["F"], t.opts(precision=prec, jitter=1, spp=2))
+    assert st.samples == int(ns.sum())
+    refined = (ns > 2).mean()
+    print(f"prec {prec}: {refined:.3f} of the pixels refined, counts {np.unique(ns).tolist()}")
+    assert 0.0 < refined < 0.6
+    for m in np.unique(ns):
+        plain, _ = t.render(cfg["F"], t.opts(precision=prec, jitter=1, spp=int(m)))
+        sel = ns == m
+        assert np.array_equal(img[sel], plain[sel]), m
     t.close()

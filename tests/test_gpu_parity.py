@@ -15,7 +15,7 @@ from oracle_ctypes import make_scene
 pytestmark = pytest.mark.gpu
 
 RAY_FIELDS = ["y", "th_inf", "ph_inf", "I", "trans", "psi", "fate", "ncross", "order", "steps", "accepted",
-              "flow_steps"]
+              "flow_steps", "sky_u", "sky_v"]
 
 
 def assert_rays_equal(gpu, ref, what):
@@ -24,7 +24,8 @@ def assert_rays_equal(gpu, ref, what):
     for f in RAY_FIELDS:
         a, b = gpu[f], ref[f]
         if a.dtype.kind == "f":
-            neq = (a.view(np.int64) != b.view(np.int64)).reshape(len(a), -1).any(axis=1)
+            bits = np.int64 if a.dtype.itemsize == 8 else np.int32
+            neq = (a.view(bits) != b.view(bits)).reshape(len(a), -1).any(axis=1)
         else:
             neq = (a != b).reshape(len(a), -1).any(axis=1)
         bad = np.nonzero(neq)[0]
@@ -36,8 +37,8 @@ def oracle_rays(oracle, cfg, px, py):
     return rays
 
 
-@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos", "sqrt",
-                                "div"])
+@pytest.mark.parametrize("fn", ["exp", "log", "expm1", "cbrt", "pow_m02", "sin", "cos", "atan2", "acos", "asin",
+                                "sqrt", "div"])
 def test_device_math_bit_exact(fn, oracle_bhm):
     rng = np.random.default_rng(7)
     n = 200000
@@ -48,7 +49,7 @@ def test_device_math_bit_exact(fn, oracle_bhm):
         x = np.concatenate([10.0 ** rng.uniform(-300, 300, n // 2), rng.uniform(0, 4, n // 2)])
     elif fn in ("sin", "cos"):
         x = np.concatenate([rng.uniform(-400, 400, n // 2), rng.uniform(-1e5, 1e5, n // 2)])
-    elif fn == "acos":
+    elif fn in ("acos", "asin"):
         x = rng.uniform(-1, 1, n)
     else:
         x = rng.normal(size=n) * 10.0 ** rng.uniform(-5, 5, n)
