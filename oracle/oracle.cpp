@@ -684,6 +684,13 @@ void parallel_for(int n, int nthreads, Fn f) {
 }  // namespace
 }  // extern "C++"
 
+// Sample s of pixel (px, py) (the ray render.cu:89-128 traces for it)
+void or_trace_sample(const or_scene* S, const or_frame* F, int px, int py, int s, int jitter, or_ray_out* out) {
+    double tx, ty;
+    or_sample_offsets(F, px, py, s, jitter, &tx, &ty);
+    or_trace_ray(S, F, tx, ty, out);
+}
+
 long long or_render_adaptive(const or_scene* S, const or_frame* F, int spp, int spp_max, float target_err,
                              int jitter, int nthreads, double* out_px, uint32_t* out_nsamp) {
     const int W = F->W, H = F->H, n = W * H, chunk = 8;

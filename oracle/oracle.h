@@ -129,6 +129,8 @@ void or_sky_uv(const double rot[9], double th, double ph, float* u, float* v);
 // Procedural checkerboard radiance at (u, v): nu x nv squares, bright `scale`, dark 0.1 scale
 float or_sky_checker(int nu, int nv, float scale, float u, float v);
 void or_trace_ray(const or_scene* S, const or_frame* F, double x, double yv, or_ray_out* out);
+// Sample s of pixel (px, py): or_sample_offsets then or_trace_ray
+void or_trace_sample(const or_scene* S, const or_frame* F, int px, int py, int s, int jitter, or_ray_out* out);
 
 // Frame drivers (std::thread, dynamic row scheduling). Rays: pixel list (px[i], py[i]), spp samples
 // each (jitter 0 = centre rays). out_px (nullable): per pixel mean of I[0..3] over the samples.

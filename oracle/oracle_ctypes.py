@@ -148,6 +148,8 @@ class Oracle:
         L.or_selftest.argtypes = [_D, _D, _I]
         L.or_math_eval.argtypes = [C.c_int, _D, _D, _D, C.c_int]
         L.or_sky_uv.argtypes = [_D, C.c_double, C.c_double, _F, _F]
+        L.or_trace_sample.argtypes = [C.POINTER(OrScene), C.POINTER(OrFrame), C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(OrRay)]
         L.or_sky_checker.restype = C.c_float
         L.or_sky_checker.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
 
@@ -164,6 +166,12 @@ class Oracle:
             self.lib.or_sky_uv(_dp(r), float(a), float(b), C.byref(uu), C.byref(vv))
             u.flat[i], v.flat[i] = uu.value, vv.value
         return u, v
+
+    def trace_sample(self, S, F, px, py, s, jitter=True):
+        """Per-ray record of sample s of pixel (px, py)."""
+        o = OrRay()
+        self.lib.or_trace_sample(C.byref(S), C.byref(F), px, py, s, int(jitter), C.byref(o))
+        return o
 
     def sky_checker(self, nu, nv, scale, u, v):
         return self.lib.or_sky_checker(nu, nv, scale, float(u), float(v))
