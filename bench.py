@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--modes", default=",".join(MODES))
     ap.add_argument("--no-autotune", action="store_true", help="time the persistent modes with the default launch")
     ap.add_argument("--no-mlp-train", action="store_true", help="skip MLP modes")
+    ap.add_argument("--mlp-save", default="", help="write the trained surrogate (parameters, normalisation) to this .npz")
+    ap.add_argument("--mlp-load", default="", help="load the surrogate from this .npz instead of training (profiling)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a device copy before the modes (profiling)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-sampling render")
     ap.add_argument("--batch-frames", type=int, default=4,
@@ -212,7 +214,16 @@ def main():
         modes = [m for m in modes if MODES[m][1] != bh.EMIS_MLP]
     if any(MODES[m][1] == bh.EMIS_MLP for m in modes):
         t0 = time.perf_counter()
-        rep = tracer.mlp_train()
+        if args.mlp_load:
+            z = np.load(args.mlp_load)
+            tracer.mlp_set(z["params"], z["norm"])
+            rep = {"loss_first": None, "loss_last": None}
+            setup["mlp_loaded"] = args.mlp_load
+        else:
+            rep = tracer.mlp_train()
+        if args.mlp_save:
+            p, nrm = tracer.mlp_get()
+            np.savez(args.mlp_save, params=p, norm=nrm)
         t1 = time.perf_counter()
         bake = tracer.mlp_bake()
         t2 = time.perf_counter()

@@ -131,6 +131,10 @@ typedef struct bh_trace_opts {
                          reference's error norm leaves out; it matters where the turbulence depends on
                          the emission time (grid time slices, the sheared model). 0 -> the default:
                          such scenes get a bound, others none; < 0 -> no bound */
+    int relax_after;  /* mixed: after this many attempts a ray's emission-region tolerance becomes
+                         tol_relax (rays that orbit near the photon sphere); 0 -> the default, < 0 -> never */
+    double tol_relax; /* mixed: that tolerance (0 -> the default) */
+    double relax_cap; /* mixed: and the factor on its transfer step cap (0 -> the default) */
 } bh_trace_opts;
 
 /* RenderStats (renderer.hpp:64-68) extended with the integrator counters. */

@@ -73,7 +73,8 @@ class Settings(C.Structure):
 class TraceOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("emissivity", C.c_int), ("jitter", C.c_int), ("spp", C.c_int),
                 ("tol_far", C.c_double), ("r_near", C.c_double), ("r_emit", C.c_double), ("far_fp32", C.c_int),
-                ("full_flow", C.c_int), ("engine", C.c_int), ("tol_t", C.c_double)]
+                ("full_flow", C.c_int), ("engine", C.c_int), ("tol_t", C.c_double), ("relax_after", C.c_int),
+                ("tol_relax", C.c_double), ("relax_cap", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -522,9 +523,9 @@ class Tracer:
 
     @staticmethod
     def opts(precision=PREC_FP64, emissivity=EMIS_EXACT, jitter=0, spp=1, tol_far=0.0, r_near=0.0, r_emit=0.0,
-             far_fp32=0, full_flow=0, engine=ENGINE_AUTO, tol_t=0.0):
+             far_fp32=0, full_flow=0, engine=ENGINE_AUTO, tol_t=0.0, relax_after=0, tol_relax=0.0, relax_cap=0.0):
         return TraceOpts(precision, emissivity, jitter, spp, tol_far, r_near, r_emit, far_fp32, full_flow, engine,
-                         tol_t)
+                         tol_t, relax_after, tol_relax, relax_cap)
 
     def render(self, F, opts=None):
         px = np.zeros((F.H, F.W, 4), dtype=np.float32)

@@ -753,6 +753,10 @@ double emission_radius(const bh_scene& S, const std::vector<float>& k2, float k2
 
 // Default bound on a mixed-tracer step's coordinate-time error (M) for time-dependent turbulence
 constexpr double DEFAULT_TOL_T = 1e-3;
+// Default long-ray relaxation: off until measured
+constexpr int DEFAULT_RELAX_AFTER = 0;
+constexpr double DEFAULT_TOL_RELAX = 1e-9;
+constexpr double DEFAULT_RELAX_CAP = 1.0;
 
 // Log-space emission constants and the region split of the mixed policy
 bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, const bh_trace_opts& o) {
@@ -805,6 +809,11 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
                                  c->turb_model == BH_TURB_SHEARED);
     const double tol_t = o.tol_t > 0 ? o.tol_t : (o.tol_t == 0 && time_dependent ? DEFAULT_TOL_T : 0.0);
     M.inv_tol_t = tol_t > 0 ? 1.0 / tol_t : 0.0;
+    // long-ray relaxation of the emission-region tolerance
+    M.relax_after = o.relax_after > 0 ? o.relax_after : (o.relax_after == 0 ? DEFAULT_RELAX_AFTER : 0x7fffffff);
+    M.tol_relax = o.tol_relax > 0 ? o.tol_relax : std::max(S.tol, DEFAULT_TOL_RELAX);
+    M.relax_cap = o.relax_cap > 0 ? o.relax_cap : DEFAULT_RELAX_CAP;
+    if (M.relax_after <= 0) M.relax_after = 0x7fffffff;
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);
     E.n_index = (float)S.n_index;

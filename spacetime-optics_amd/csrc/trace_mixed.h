@@ -436,6 +436,9 @@ struct MixedParams {
     double tol_mid;   // tolerance between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
     double inv_tol_t; // 1 / (absolute bound on a step's coordinate-time error); 0: no bound
+    double tol_relax; // emission-region tolerance of a ray after relax_after attempts
+    double relax_cap; // and the factor on its transfer step cap
+    int relax_after;
     EmisParams E;
     int use_surrogate;
     float ln_e2_over_c, ln_e2_over_mec;   // ln(e^2 / c), ln(e^2 / (m_e c))
@@ -482,7 +485,8 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     double hc = R.h;
     if (flow && R.s.y0 < M.r_cut) {
         double sn = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0));
-        double cap = P.step_frac * fmin(R.s.y0 * rcpd(fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(R.s.y3) + 1e-30));
+        const double frac = R.it >= M.relax_after ? P.step_frac * M.relax_cap : P.step_frac;
+        double cap = frac * fmin(R.s.y0 * rcpd(fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(R.s.y3) + 1e-30));
         hc = fmin(hc, cap);
     }
     double err;
@@ -495,7 +499,10 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
             Rhs<double> k = rhs<double>(cd, R.s.y0, R.s.y2);
             R.s.k1 = k.d1; R.s.k3 = k.d3; R.s.k4 = k.d4; R.s.k5 = k.d5;
         }
-        const double tol = R.s.y0 < M.r_tol ? P.tol : (R.s.y0 < M.r_far ? M.tol_mid : M.tol_far);
+        // inside the emission region the scene tolerance, relaxed for a ray that has already taken
+        // relax_after attempts (an orbit near the photon sphere)
+        const double tol = R.s.y0 < M.r_tol ? (R.it >= M.relax_after ? M.tol_relax : P.tol)
+                                            : (R.s.y0 < M.r_far ? M.tol_mid : M.tol_far);
         err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, M.inv_tol_t, n);
         ++R.fp64_attempts;
     }

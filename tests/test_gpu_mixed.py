@@ -56,3 +56,24 @@ def test_mixed_vacuum_c1_fates(far_fp32):
     assert same_fate > 0.999
     assert same_n > 0.995
     assert np.median(dth) < 1e-5
+
+
+def test_long_ray_relaxation():
+    """The optional long-ray rule (after relax_after attempts a ray's emission-region tolerance is
+    tol_relax and its transfer step cap relax_cap times the scene's): the longest rays get shorter and
+    the frame stays within the bars."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=512, H=512)
+    t = tracer_for(cfg)
+    ref, _ = t.render(cfg["F"], bh.Tracer.opts(precision=bh.PREC_FP64))
+    base = t.trace_tile(cfg["F"], 0, 0, 512, 512, bh.Tracer.opts(precision=bh.PREC_MIXED)).ravel()
+    o = bh.Tracer.opts(precision=bh.PREC_MIXED, relax_after=300, tol_relax=1e-9, relax_cap=3.0)
+    rel = t.trace_tile(cfg["F"], 0, 0, 512, 512, o).ravel()
+    img, _ = t.render(cfg["F"], o)
+    flux, med, p99 = frame_errors(img, ref)
+    print(f"long-ray relaxation: max attempts {base['steps'].max()} -> {rel['steps'].max()}; flux {flux:.2e} "
+          f"median {med:.2e} p99 {p99:.2e}")
+    assert rel["steps"].max() < base["steps"].max()
+    short = base["steps"] < 300
+    assert np.array_equal(rel["I"][short], base["I"][short])   # rays below the threshold are untouched
+    assert flux < 1e-3 and med < 1e-2
+    t.close()

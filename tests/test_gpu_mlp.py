@@ -101,16 +101,30 @@ def test_training_converges_and_is_accurate(ctx, trained):
     assert np.median(dj) < 2e-3
 
 
+def median_kernel_ms(t, F, opts, n=7):
+    ms = []
+    for _ in range(n):
+        img, st = t.render(F, opts)
+        ms.append(st.ms_kernel)
+    return img, st, float(np.median(ms[1:]))
+
+
 def test_surrogate_in_tracer_and_gain(ctx, trained):
-    """What the trained network gains: mixed tracer with the baked surrogate vs the exact FP32
-    formula and vs the plain FP64 tracer on the 1024^2 frame."""
+    """What the trained network gains over the method without it (the mixed tracer with the exact
+    FP32 formula) on the 1024^2 frame: nothing. The table lookup is a memory round trip that waits
+    for the turbulence lookup (the surrogate's coordinate X depends on the density), while the exact
+    routine costs three hardware transcendentals and a K2 lookup that does not wait; even with every
+    table cell trusted (no exact fallback at all) the surrogate frame is slower. Pinned here: the
+    surrogate frame takes 0.97 - 1.3 x the exact frame's kernel time (measured 1.07 - 1.09 x), and
+    the network's own inference costs more than 5 x the exact routine per sample (measured ~20 x:
+    1.7 ns against 0.087 ns for the FP64 exact formula)."""
     t, cfg = ctx
     bake = t.mlp_bake()
     print(f"bake: {bake}")
     F = cfg["F"]
-    ref, st64 = t.render(F, bh.Tracer.opts(precision=bh.PREC_FP64))
-    mix, stx = t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED))
-    mlp, stm = t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_MLP))
+    ref, st64, ms64 = median_kernel_ms(t, F, bh.Tracer.opts(precision=bh.PREC_FP64), n=3)
+    mix, stx, msx = median_kernel_ms(t, F, bh.Tracer.opts(precision=bh.PREC_MIXED))
+    mlp, stm, msm = median_kernel_ms(t, F, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_MLP))
     Ib = ref[..., 0].astype(np.float64)
     mask = Ib > 1e-3 * Ib.max()
     for name, img, st in (("mixed+exact", mix, stx), ("mixed+mlp", mlp, stm)):
@@ -119,8 +133,14 @@ def test_surrogate_in_tracer_and_gain(ctx, trained):
         med = np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask])
         print(f"{name}: flux err {flux:.3e}, pixel median {med:.3e}, kernel {st.ms_kernel:.2f} ms "
               f"(fp64 {st64.ms_kernel:.2f} ms, speed-up {st64.ms_kernel / st.ms_kernel:.2f}x)")
-    print(f"surrogate vs exact emission inside the mixed tracer: {stx.ms_kernel / stm.ms_kernel:.3f}x")
+    ratio = msm / msx
+    infer_ns = bake["ms_bake"] * 1e6 / bake["cells"]            # network inference per sample
+    exact_ns = bake["ms_judge"] * 1e6 / (4 * bake["cells"])     # exact FP64 formula per evaluation
+    print(f"surrogate vs exact emission inside the mixed tracer: {ratio:.3f} x the kernel time "
+          f"({msm:.3f} vs {msx:.3f} ms); inference {infer_ns:.2f} ns per sample vs exact {exact_ns:.3f} ns")
     Ia = mlp[..., 0].astype(np.float64)
     assert abs(Ia.sum() - Ib.sum()) / Ib.sum() < 1e-3
     assert np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask]) < 1e-2
-    assert stm.ms_kernel < st64.ms_kernel
+    assert 0.97 <= ratio < 1.3   # no gain; the lower bound allows for launch-to-launch spread
+    assert infer_ns > 5 * exact_ns
+    assert msm < ms64
