@@ -124,7 +124,8 @@ typedef struct bh_trace_opts {
     int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> the persistent engine
                          (bh_autotune's last result for this precision, or the defaults): one sample per
                          pixel as bh_trace_batch of one frame, more as the first pass of
-                         bh_render_adaptive alone; frames beyond 2^31 samples on the frame kernel.
+                         bh_render_adaptive alone; frames beyond 2^31 samples, or whose samples
+                         would take more than 1 GiB (32 bytes each), on the frame kernel.
                          BH_ENGINE_FRAME -> always the frame kernel. Both give the same pixels and
                          counters. */
     double tol_t;     /* mixed: absolute bound (M) on one step's coordinate-time error, which the

@@ -82,6 +82,12 @@ int bhh_save_font(const char* path, const bh_font* font);
    (nullable) and returns the count. */
 int bhh_abi_sizes(size_t* out, int n);
 
+/* offsetof of every field of the same structs, struct by struct in that order and field by field in
+   declaration order (an array or nested struct counts as one field), so a binding can check each
+   field's place, not only each struct's size. No counterpart in the reference. Writes min(n, count)
+   offsets to out (nullable) and returns the count. */
+int bhh_abi_offsets(size_t* out, int n);
+
 /* Self-test rows (8 x 6: a, incl_deg, traced -, traced +, analytic -, analytic +) on `device`. */
 int bhh_selftest(int device, double* rows, char* err, size_t errlen);
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "bhh_create_from_ini", "bhh_find_critical_point", "bhh_selftest", "bhh_write_png", "bhh_smooth_exposure",
     "bh_mlp_train", "bh_mlp_get", "bh_mlp_set", "bh_mlp_infer", "bh_mlp_exact", "bh_mlp_grad", "bh_mlp_bake",
     "bh_trace_batch", "bh_autotune", "bh_render_adaptive", "bh_post_radio", "bh_tonemap", "bh_sheared_slices",
-    "bh_tonemap_hud", "bhh_make_hud", "bhh_save_font", "bhh_abi_sizes", "bh_render_noalloc",
+    "bh_tonemap_hud", "bhh_make_hud", "bhh_save_font", "bhh_abi_sizes", "bh_render_noalloc", "bhh_abi_offsets",
 ]
 MLP_NPARAM = 8706
 SHOT_VALUES = 20   # values per frame of bhh_build_shot
@@ -259,6 +259,8 @@ def load():
     L.bhh_save_font.argtypes = [C.c_char_p, C.POINTER(Font)]
     L.bhh_abi_sizes.restype = C.c_int
     L.bhh_abi_sizes.argtypes = [C.POINTER(C.c_size_t), C.c_int]
+    L.bhh_abi_offsets.restype = C.c_int
+    L.bhh_abi_offsets.argtypes = [C.POINTER(C.c_size_t), C.c_int]
     L.bhh_selftest.restype = C.c_int
     L.bhh_selftest.argtypes = [C.c_int, _D, C.c_char_p, C.c_size_t]
     _lib = L

@@ -976,6 +976,7 @@ int persist_grid(const bh_ctx* c, const void* k, int block, int grid_per_cu) {
     return c->n_cu * grid_per_cu;
 }
 
+constexpr size_t PLAIN_SPP_SAMPLE_BUDGET = size_t(1) << 30;   // bytes of per-sample storage of bh_render, spp > 1
 constexpr long long BATCH_MAX_PIXELS = 1ll << 30;   // per frame: keeps the 32-bit queue counters from wrapping
 
 // Defaults: the autotuner's choices on the 1024^2 230 GHz frame (MI355X). The mixed tracer's launch is
@@ -1343,11 +1344,14 @@ int render_frame(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, const 
     if (opts) o = *opts;
     const int spp = o.spp > 0 ? o.spp : (f->spp > 0 ? f->spp : 1);
     const long long entries = (long long)((f->W + 7) / 8) * ((f->H + 7) / 8) * 64 * spp;
+    // the persistent path keeps every sample (FP64 x 4) until the per-pixel sums; beyond this budget
+    // the frame kernel, which sums a pixel's samples in registers, gives the same pixels
+    const bool samples_fit = (double)entries * sizeof(double4) <= (double)PLAIN_SPP_SAMPLE_BUDGET;
     if (o.engine == BH_ENGINE_AUTO && spp == 1 && (long long)f->W * f->H <= BATCH_MAX_PIXELS) {
         const bh_batch_frame bf{c->S, *f};
         rc = launch_batch(c, &bf, 1, &o, nullptr, nullptr, st);
         *d_out = c->d_bout;
-    } else if (o.engine == BH_ENGINE_AUTO && spp > 1 && entries < (1ll << 31)) {
+    } else if (o.engine == BH_ENGINE_AUTO && spp > 1 && entries < (1ll << 31) && samples_fit) {
         rc = launch_adaptive(c, f, &o, nullptr, st, spp);
         *d_out = c->d_px;
     } else {

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <string>
 
+#include <cstddef>
 #include "../../../include/bhrt_host.h"
 #include "config.h"
 #include "hud.h"
@@ -262,6 +263,82 @@ int bhh_abi_sizes(size_t* out, int n) {
                          sizeof(bh_hud_rect),     sizeof(bh_font)};
     const int count = (int)(sizeof sz / sizeof sz[0]);
     for (int i = 0; out && i < n && i < count; ++i) out[i] = sz[i];
+    return count;
+}
+
+int bhh_abi_offsets(size_t* out, int n) {
+#define BH_OFF(T, f) offsetof(T, f)
+    const size_t off[] = {
+        // bh_scene
+        BH_OFF(bh_scene, a), BH_OFF(bh_scene, rh), BH_OFF(bh_scene, r_isco), BH_OFF(bh_scene, tol),
+        BH_OFF(bh_scene, max_steps), BH_OFF(bh_scene, r_escape), BH_OFF(bh_scene, r_stop), BH_OFF(bh_scene, step_frac),
+        BH_OFF(bh_scene, mode), BH_OFF(bh_scene, flow_on), BH_OFF(bh_scene, n0), BH_OFF(bh_scene, n_index),
+        BH_OFF(bh_scene, theta_index), BH_OFF(bh_scene, theta0), BH_OFF(bh_scene, h_r), BH_OFF(bh_scene, beta_p),
+        BH_OFF(bh_scene, r_flow_max), BH_OFF(bh_scene, nu_obs), BH_OFF(bh_scene, rg_cm), BH_OFF(bh_scene, flow_kep),
+        BH_OFF(bh_scene, flow_turb_amp), BH_OFF(bh_scene, seed), BH_OFF(bh_scene, flow_turb_scale),
+        BH_OFF(bh_scene, flow_tcorr), BH_OFF(bh_scene, sky_on), BH_OFF(bh_scene, sky_nu), BH_OFF(bh_scene, sky_nv),
+        BH_OFF(bh_scene, sky_scale), BH_OFF(bh_scene, sky_rot),
+        // bh_tables
+        BH_OFF(bh_tables, k2_ln), BH_OFF(bh_tables, k2_n), BH_OFF(bh_tables, k2_l0), BH_OFF(bh_tables, k2_dl),
+        BH_OFF(bh_tables, turb), BH_OFF(bh_tables, turb_n), BH_OFF(bh_tables, turb_nt), BH_OFF(bh_tables, turb_half),
+        BH_OFF(bh_tables, turb_t0), BH_OFF(bh_tables, turb_dt), BH_OFF(bh_tables, turb_model), BH_OFF(bh_tables, ts_dims),
+        // bh_frame
+        BH_OFF(bh_frame, r), BH_OFF(bh_frame, th), BH_OFF(bh_frame, ph), BH_OFF(bh_frame, t), BH_OFF(bh_frame, kp),
+        BH_OFF(bh_frame, F), BH_OFF(bh_frame, R), BH_OFF(bh_frame, U), BH_OFF(bh_frame, beta), BH_OFF(bh_frame, cx),
+        BH_OFF(bh_frame, cy), BH_OFF(bh_frame, pix), BH_OFF(bh_frame, W), BH_OFF(bh_frame, H), BH_OFF(bh_frame, spp),
+        BH_OFF(bh_frame, seed),
+        // bh_settings
+        BH_OFF(bh_settings, W), BH_OFF(bh_settings, H), BH_OFF(bh_settings, spp), BH_OFF(bh_settings, spp_max),
+        BH_OFF(bh_settings, target_err),
+        // bh_trace_opts
+        BH_OFF(bh_trace_opts, precision), BH_OFF(bh_trace_opts, emissivity), BH_OFF(bh_trace_opts, jitter),
+        BH_OFF(bh_trace_opts, spp), BH_OFF(bh_trace_opts, tol_far), BH_OFF(bh_trace_opts, r_near),
+        BH_OFF(bh_trace_opts, r_emit), BH_OFF(bh_trace_opts, far_fp32), BH_OFF(bh_trace_opts, full_flow),
+        BH_OFF(bh_trace_opts, engine), BH_OFF(bh_trace_opts, tol_t), BH_OFF(bh_trace_opts, relax_after),
+        BH_OFF(bh_trace_opts, tol_relax), BH_OFF(bh_trace_opts, relax_cap),
+        // bh_stats
+        BH_OFF(bh_stats, ms_trace), BH_OFF(bh_stats, ms_kernel), BH_OFF(bh_stats, samples), BH_OFF(bh_stats, step_limited),
+        BH_OFF(bh_stats, step_attempts), BH_OFF(bh_stats, steps_accepted), BH_OFF(bh_stats, flow_steps),
+        BH_OFF(bh_stats, fp64_steps), BH_OFF(bh_stats, flux_sum),
+        // bh_ray_out
+        BH_OFF(bh_ray_out, y), BH_OFF(bh_ray_out, th_inf), BH_OFF(bh_ray_out, ph_inf), BH_OFF(bh_ray_out, I),
+        BH_OFF(bh_ray_out, trans), BH_OFF(bh_ray_out, psi), BH_OFF(bh_ray_out, fate), BH_OFF(bh_ray_out, ncross),
+        BH_OFF(bh_ray_out, order), BH_OFF(bh_ray_out, steps), BH_OFF(bh_ray_out, accepted),
+        BH_OFF(bh_ray_out, flow_steps), BH_OFF(bh_ray_out, sky_u), BH_OFF(bh_ray_out, sky_v),
+        // bh_batch_frame
+        BH_OFF(bh_batch_frame, scene), BH_OFF(bh_batch_frame, frame),
+        // bh_batch_cfg
+        BH_OFF(bh_batch_cfg, block), BH_OFF(bh_batch_cfg, grid_per_cu), BH_OFF(bh_batch_cfg, interval),
+        BH_OFF(bh_batch_cfg, refill_min), BH_OFF(bh_batch_cfg, use_graph), BH_OFF(bh_batch_cfg, waves),
+        BH_OFF(bh_batch_cfg, order), BH_OFF(bh_batch_cfg, prio_age),
+        // bh_mlp_train_cfg
+        BH_OFF(bh_mlp_train_cfg, n_samples), BH_OFF(bh_mlp_train_cfg, batch), BH_OFF(bh_mlp_train_cfg, epochs),
+        BH_OFF(bh_mlp_train_cfg, lr), BH_OFF(bh_mlp_train_cfg, lr_min), BH_OFF(bh_mlp_train_cfg, hot_weight),
+        BH_OFF(bh_mlp_train_cfg, seed),
+        // bh_mlp_report
+        BH_OFF(bh_mlp_report, ms_sample), BH_OFF(bh_mlp_report, ms_train), BH_OFF(bh_mlp_report, steps),
+        BH_OFF(bh_mlp_report, loss_first), BH_OFF(bh_mlp_report, loss_last), BH_OFF(bh_mlp_report, train_flop),
+        // bh_mlp_bake_report
+        BH_OFF(bh_mlp_bake_report, ms_bake), BH_OFF(bh_mlp_bake_report, ms_judge), BH_OFF(bh_mlp_bake_report, cells),
+        BH_OFF(bh_mlp_bake_report, hot_cells), BH_OFF(bh_mlp_bake_report, failed_hot_cells),
+        // bh_post_params
+        BH_OFF(bh_post_params, blur_sigma_px), BH_OFF(bh_post_params, hide_direct),
+        // bh_tone_params
+        BH_OFF(bh_tone_params, colormap), BH_OFF(bh_tone_params, exposure), BH_OFF(bh_tone_params, gamma),
+        BH_OFF(bh_tone_params, overlay), BH_OFF(bh_tone_params, hide_direct), BH_OFF(bh_tone_params, fade),
+        BH_OFF(bh_tone_params, vignette), BH_OFF(bh_tone_params, grain), BH_OFF(bh_tone_params, seed),
+        // bh_hud_glyph
+        BH_OFF(bh_hud_glyph, x), BH_OFF(bh_hud_glyph, y), BH_OFF(bh_hud_glyph, scale), BH_OFF(bh_hud_glyph, glyph),
+        BH_OFF(bh_hud_glyph, rgba),
+        // bh_hud_rect
+        BH_OFF(bh_hud_rect, x0), BH_OFF(bh_hud_rect, y0), BH_OFF(bh_hud_rect, x1), BH_OFF(bh_hud_rect, y1),
+        BH_OFF(bh_hud_rect, rgba), BH_OFF(bh_hud_rect, layer),
+        // bh_font
+        BH_OFF(bh_font, atlas), BH_OFF(bh_font, cell_w), BH_OFF(bh_font, cell_h), BH_OFF(bh_font, glyphs),
+        BH_OFF(bh_font, advance), BH_OFF(bh_font, baseline)};
+#undef BH_OFF
+    const int count = (int)(sizeof off / sizeof off[0]);
+    for (int i = 0; out && i < n && i < count; ++i) out[i] = off[i];
     return count;
 }
 

@@ -209,3 +209,18 @@ def test_batched_animation_shot_reduced(prec):
     assert np.array_equal(stats_vec(st), cnt)
     assert abs(st.flux_sum - flux) <= 1e-12 * abs(flux)
     t.close()
+
+
+def test_render_many_samples_beyond_the_sample_budget():
+    """bh_render keeps every sample of a frame with spp > 1 on the persistent engine only while they
+    fit 1 GiB (32 bytes each); 2048^2 pixels x 9 samples (1.2 GB) run on the frame kernel instead,
+    with the same pixels as engine BH_ENGINE_FRAME."""
+    cfg = setup(shot("c2_radio_1024.ini"), W=2048, H=2048)
+    t = tracer_for(cfg)
+    o = t.opts(precision=bh.PREC_MIXED, jitter=1, spp=9)
+    assert 2048 * 2048 * 9 * 32 > 1 << 30
+    got, st = t.render(cfg["F"], o)
+    want, sf = t.render(cfg["F"], t.opts(precision=bh.PREC_MIXED, jitter=1, spp=9, engine=bh.ENGINE_FRAME))
+    assert_same([got], [want], "spp 9 at 2048^2")
+    assert np.array_equal(stats_vec(st), stats_vec(sf)) and st.samples == 9 * 2048 * 2048
+    t.close()

@@ -119,8 +119,9 @@ def test_c_abi_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_the_library():
-    """Every ctypes mirror of a public struct has the C compiler's size (a field missing, added or
-    of the wrong type on either side changes it)."""
+    """Every ctypes mirror of a public struct has the C compiler's size, and every field sits at the
+    C compiler's offset (bhh_abi_offsets, struct by struct, fields in declaration order): a missing,
+    added, reordered or wrongly sized field on either side fails."""
     L = bh.load()
     mirrors = [bh.Scene, bh.Tables, bh.Frame, bh.Settings, bh.TraceOpts, bh.Stats, bh.RAY_DTYPE, bh.BatchFrame,
                bh.BatchCfg, bh.MlpTrainCfg, bh.MlpReport, bh.MlpBakeReport, bh.PostParams, bh.ToneParams,
@@ -132,6 +133,18 @@ def test_struct_layouts_match_the_library():
     for m, got in zip(mirrors, sizes):
         want = m.itemsize if isinstance(m, np.dtype) else C.sizeof(m)
         assert got == want, (getattr(m, "__name__", "RAY_DTYPE"), got, want)
+    want = []
+    for m in mirrors:
+        if isinstance(m, np.dtype):
+            want += [(m.names[i], m.fields[f][1]) for i, f in enumerate(m.names)]
+        else:
+            want += [(m.__name__ + "." + name, getattr(m, name).offset) for name, _ in m._fields_]
+    n = L.bhh_abi_offsets(None, 0)
+    offs = (C.c_size_t * n)()
+    assert L.bhh_abi_offsets(offs, n) == n
+    assert n == len(want), (n, len(want))
+    for (name, w), got in zip(want, offs):
+        assert got == w, (name, got, w)
 
 
 def test_error_paths_without_gpu():
