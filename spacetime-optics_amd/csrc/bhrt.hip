@@ -820,6 +820,7 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
     E.ln_theta0 = (float)std::log(S.theta0);
     E.theta_index = (float)S.theta_index;
     E.h_r = (float)S.h_r;
+    E.inv_h_r = (float)(1.0 / S.h_r);
     E.r_flow_max = (float)S.r_flow_max;
     E.rh = (float)S.rh;
     E.r_isco = (float)S.r_isco;

@@ -54,21 +54,6 @@ __device__ __forceinline__ double rcpd(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-// atan2(y, x) for y >= 0, in [0, pi]: octant reduction and t * P(t^2) on [0, 1] with P(0) = 1
-// (least-squares fit to atan on Chebyshev nodes, absolute error <= 4.0e-7, relative <= 1.6e-6)
-__device__ __forceinline__ float atan2_pos(float y, float x) {
-    const float ax = fabsf(x), mx = fmaxf(ax, y), mn = fminf(ax, y);
-    const float t = mx > 0.f ? mn * rcpf(mx) : 0.f, t2 = t * t;
-    float q = 0.0073739392682909966f;
-    q = q * t2 - 0.03551968187093735f;
-    q = q * t2 + 0.08216753602027893f;
-    q = q * t2 - 0.13398787379264832f;
-    q = q * t2 + 0.19861853122711182f;
-    q = q * t2 - 0.3332539498806f;
-    float a = t + t * t2 * q;
-    a = y > ax ? 1.5707963267948966f - a : a;
-    return x < 0.f ? 3.141592653589793f - a : a;
-}
 template <typename R>
 __device__ __forceinline__ R rcp(R x);
 template <>
@@ -81,17 +66,53 @@ __device__ __forceinline__ float lnf(float x) { return __builtin_amdgcn_logf(x) 
 __device__ __forceinline__ float expf_(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
 // x^-0.2 in FP64 (the step-size controller's factor) for x clamped to [1e-30, 1e30]: an FP32 seed from
-// the hardware log2 / exp2 (relative error ~1e-6), then two Newton steps on x y^5 = 1,
-// y <- y + y (1 - x y^5) / 5, each squaring the relative error (times 2): 1e-6 -> 2e-12 -> FP64 rounding.
+// the hardware log2 / exp2 (relative error ~2e-7), then one Newton step on x y^5 = 1,
+// y <- y + y (1 - x y^5) / 5, which squares the relative error (times 3): ~1e-13.
 // A NaN x gives the factor of 1e-30 (the callers handle a NaN error themselves).
 __device__ __forceinline__ double pow_m02_d(double x) {
     const double xc = fmin(fmax(x, 1e-30), 1e30);
-    double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xc));
-    for (int i = 0; i < 2; ++i) {
-        const double y2 = y * y, y5 = y2 * y2 * y;
-        y = fma(y * 0.2, fma(-xc, y5, 1.0), y);
+    const double y = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)xc));
+    const double y2 = y * y, y5 = y2 * y2 * y;
+    return fma(y * 0.2, fma(-xc, y5, 1.0), y);
+}
+
+// FP64 square root of x >= 0 from the hardware reciprocal square root and two Goldschmidt
+// refinements (a few ulp; the correctly rounded sqrt adds scaling and fix-up for ranges the tracer
+// never produces: x is a radial potential of order r^4, far from the denormals)
+__device__ __forceinline__ double sqrt_d(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, x);
+    return x > 0.0 ? fma(d, h, g) : 0.0;
+}
+
+// blackholes/src/kerr.hpp:166-173 with the square root above
+__device__ __forceinline__ void project_radial_m(const TraceParams& P, const RayConst& c, State& s) {
+    const double r = s.y0;
+    const double Pp = r * r + P.a2 - c.alam, D = r * r - 2.0 * r + P.a2;
+    const double R = Pp * Pp - D * c.K;
+    if (R > 0.25 * r * r * r * r) {
+        const double v = sqrt_d(R);
+        s.y1 = s.y1 < 0.0 ? -v : v;
     }
-    return y;
+}
+
+// asin on [0, 1] in FP32: t + t^3 P(t^2) on [0, 1/2] (P of degree 4, least squares on Chebyshev
+// nodes for the absolute error: 1.6e-9, 4.7e-8 with FP32 evaluation), and
+// pi/2 - 2 asin(sqrt((1 - x) / 2)) above 1/2
+__device__ __forceinline__ float asin_pos(float x) {
+    const bool big = x > 0.5f;
+    const float t = big ? sqrtf_(0.5f * (1.f - x)) : x, z = t * t;
+    float p = 0.0437449403f;
+    p = p * z + 0.0231502242f;
+    p = p * z + 0.0457071587f;
+    p = p * z + 0.0749306679f;
+    p = p * z + 0.166668221f;
+    const float a = t + t * z * p;
+    return big ? 1.5707963267948966f - 2.f * a : a;
 }
 
 template <typename R>
@@ -219,7 +240,7 @@ __device__ __forceinline__ RConst<R> make_const(const TraceParams& P, const RayC
 // Emission (blackholes/src/emission.cuh:99-175) in FP32, log space.
 struct EmisParams {
     // log-space constants, filled on the host
-    float ln_n0, n_index, ln_theta0, theta_index, h_r, r_flow_max, rh, r_isco, flow_kep, a, a2;
+    float ln_n0, n_index, ln_theta0, theta_index, h_r, inv_h_r, r_flow_max, rh, r_isco, flow_kep, a, a2;
     float ln_bconst;   // ln(8 pi m_p c^2 / (6 beta)) : B^2 = exp(ln_bconst) n_e / r
     float ln_nuc_per_B;   // ln(e / (2 pi m_e c))
     float ln_jconst;   // ln(sqrt(2) pi e^2 / (3 c))
@@ -254,51 +275,84 @@ __device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const Em
     *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + lnX * (1.f / 3.f) + 2.f * lnf(y + 1.887748625363387f) - y;
     // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x)); ln x is known
     const float ln_x = E.ln_h_over_mec2 + ln_nu - ln_th;
-    const float x = expf_(ln_x);
-    const float ln_em1 = x < 1e-3f ? ln_x + 0.5f * x : (x > 30.f ? x : lnf(expf_(x) - 1.f));
+    // ln(expm1(x)) = ln x + x/2 + ...; for ln x < -15 the x/2 term is below half an FP32 ulp of ln x,
+    // so the sum is ln x itself and x need not be formed (230 GHz in this flow: x ~ 1e-9)
+    float ln_em1 = ln_x;
+    if (ln_x >= -15.f) {
+        const float x = expf_(ln_x);
+        ln_em1 = x < 1e-3f ? ln_x + 0.5f * x : (x > 30.f ? x : lnf(expf_(x) - 1.f));
+    }
     *ln_bnu = E.ln_bnu_const + 3.f * ln_nu - ln_em1;
     *ln_a = *ln_j - *ln_bnu;
 }
 
 // Turbulence density factor in FP32: the grid, cell-centred trilinear weights, edge clamps and
 // time-slice interpolation of bhd::turb_factor, with the azimuth reduced to [0, 2 pi) in FP64 before
-// the FP32 sine and cosine (an orbiting photon's phi grows by 2 pi per turn).
-__device__ __forceinline__ float grid_trilinear_f32(const TraceParams& P, const float* __restrict__ G, float px,
-                                                    float py, float pz) {
+// the FP32 sine and cosine (an orbiting photon's phi grows by 2 pi per turn). Split in two: the fetch
+// computes the cell and issues the loads of its corners (one slice, or the two slices around the
+// emission time); the blend interpolates them. Between the two the tracer computes what does not
+// depend on the density, so the loads' latency is hidden behind that work.
+struct Corners {
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+};
+struct TurbFetch {
+    Corners k0, k1;   // slice k (and k + 1)
+    float fx, fy, fz, ft;
+    bool two;
+};
+
+__device__ __forceinline__ Corners grid_corners_f32(const float* __restrict__ b, unsigned int sy, unsigned int sz) {
+    return Corners{b[0], b[1], b[sy], b[sy + 1], b[sz], b[sz + 1], b[sz + sy], b[sz + sy + 1]};
+}
+
+__device__ __forceinline__ float grid_blend_f32(const Corners& v, float fx, float fy, float fz) {
+    const float c00 = v.c000 + (v.c100 - v.c000) * fx, c10 = v.c010 + (v.c110 - v.c010) * fx;
+    const float c01 = v.c001 + (v.c101 - v.c001) * fx, c11 = v.c011 + (v.c111 - v.c011) * fx;
+    const float c0 = c00 + (c10 - c00) * fy, c1 = c01 + (c11 - c01) * fy;
+    return c0 + (c1 - c0) * fz;
+}
+
+__device__ __forceinline__ void turb_fetch_f32(const TraceParams& P, float R, float r, float u, double phid,
+                                               double td, TurbFetch& T) {
+    const double two_pi = 6.283185307179586;
+    const float ph = (float)(phid - two_pi * floor(phid * (1.0 / two_pi)));
+    float sp, cp;
+    __sincosf(ph, &sp, &cp);
+    const float px = R * cp, py = R * sp, pz = r * u;
     const int n = P.turb_n;
     const float half = (float)P.turb_half, inv = (float)P.turb_inv_dx, hi = (float)(n - 1);
     const float gx = fminf(fmaxf((px + half) * inv - 0.5f, 0.f), hi);
     const float gy = fminf(fmaxf((py + half) * inv - 0.5f, 0.f), hi);
     const float gz = fminf(fmaxf((pz + half) * inv - 0.5f, 0.f), hi);
     const int ix = min((int)gx, n - 2), iy = min((int)gy, n - 2), iz = min((int)gz, n - 2);
-    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    T.fx = gx - (float)ix;
+    T.fy = gy - (float)iy;
+    T.fz = gz - (float)iz;
     // 32-bit offsets: bh_create keeps the grid (n^3 cells x nt slices) below 2^31 cells
     const unsigned int sy = (unsigned int)n, sz = (unsigned int)(n * n);
-    const float* b = G + ((unsigned int)iz * sz + (unsigned int)iy * sy + (unsigned int)ix);
-    const float v000 = b[0], v100 = b[1], v010 = b[sy], v110 = b[sy + 1];
-    const float v001 = b[sz], v101 = b[sz + 1], v011 = b[sz + sy], v111 = b[sz + sy + 1];
-    const float c00 = v000 + (v100 - v000) * fx, c10 = v010 + (v110 - v010) * fx;
-    const float c01 = v001 + (v101 - v001) * fx, c11 = v011 + (v111 - v011) * fx;
-    const float c0 = c00 + (c10 - c00) * fy, c1 = c01 + (c11 - c01) * fy;
-    return c0 + (c1 - c0) * fz;
+    const unsigned int cell = (unsigned int)iz * sz + (unsigned int)iy * sy + (unsigned int)ix;
+    T.two = P.turb_nt > 1;
+    // the slice (pair) and its weight first; the corner loads then write the same fields on both
+    // paths, so the struct stays in registers (no store through a selected address)
+    const float* G = P.turb + cell;
+    unsigned int slice = 0;
+    T.ft = 0.f;
+    if (T.two) {
+        const float st = fminf(fmaxf((float)((td - P.turb_t0) * P.turb_inv_dt), 0.f), (float)(P.turb_nt - 1));
+        const int k = min((int)st, P.turb_nt - 2);
+        T.ft = st - (float)k;
+        slice = (unsigned int)(P.turb_n * P.turb_n * P.turb_n);
+        G += (unsigned int)k * slice;
+    }
+    T.k0 = grid_corners_f32(G, sy, sz);
+    if (T.two) T.k1 = grid_corners_f32(G + slice, sy, sz);   // read only when two (a copy would wait for k0)
 }
 
-__device__ __forceinline__ float turb_factor_f32(const TraceParams& P, float r, float u, double phid, double td) {
-    const float R = r * sqrtf_(fmaxf(1.f - u * u, 1e-12f));
-    const double two_pi = 6.283185307179586;
-    const float ph = (float)(phid - two_pi * floor(phid * (1.0 / two_pi)));
-    float sp, cp;
-    __sincosf(ph, &sp, &cp);
-    const float px = R * cp, py = R * sp, pz = r * u;
-    if (P.turb_nt <= 1) return grid_trilinear_f32(P, P.turb, px, py, pz);
-    const float st = fminf(fmaxf((float)((td - P.turb_t0) * P.turb_inv_dt), 0.f), (float)(P.turb_nt - 1));
-    const int k = min((int)st, P.turb_nt - 2);
-    const float f = st - (float)k;
-    const unsigned int slice = (unsigned int)(P.turb_n * P.turb_n * P.turb_n);
-    const float* G = P.turb + (unsigned int)k * slice;
-    const float A = grid_trilinear_f32(P, G, px, py, pz);
-    const float B = grid_trilinear_f32(P, G + slice, px, py, pz);
-    return A + (B - A) * f;
+__device__ __forceinline__ float turb_blend_f32(const TurbFetch& T) {
+    const float A = grid_blend_f32(T.k0, T.fx, T.fy, T.fz);
+    if (!T.two) return A;
+    const float B = grid_blend_f32(T.k1, T.fx, T.fy, T.fz);
+    return A + (B - A) * T.ft;
 }
 
 // The sheared turbulence in FP32 (same slices and filter as bhd::sheared_factor): slice coordinates
@@ -320,45 +374,49 @@ __device__ __forceinline__ float sheared_factor_f32(const TraceParams& P, float 
     return A + (B - A) * f;
 }
 
-// Geometry + plasma of one flow sample; returns false when nothing is emitted or absorbed.
+// Geometry + plasma of one flow sample, in two parts. flow_begin takes everything that does not
+// depend on the density - the vertical profile, the horizon softening, the fluid four-velocity, the
+// Doppler factor, the pitch angle, the path length - and issues the turbulence loads (grid model);
+// flow_finish blends them into the density and the cyclotron frequency. Each returns false when the
+// sample emits and absorbs nothing; the early-outs are those of emission.cuh:99-175.
 struct FlowSample {
     float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl;
 };
+struct FlowPending {
+    float lr, lnvert, soft, R, z;
+    TurbFetch tf;
+};
 
 template <int RADIO>
-__device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisParams& E, double rd, double ud,
-                                            double phid, double td, double lamd, double Ecamd, double hd,
-                                            FlowSample& f) {
+__device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParams& E, double rd, double ud,
+                                           double phid, double td, double lamd, double Ecamd, double hd,
+                                           FlowSample& f, FlowPending& q) {
     const float r = (float)rd, u = (float)ud, lam = (float)lamd;
-    const float s2 = fmaxf(1.f - u * u, 1e-12f), s = sqrtf_(s2);
+    // sin theta and its reciprocal from one reciprocal square root
+    const float s2 = fmaxf(1.f - u * u, 1e-12f), is = __builtin_amdgcn_rsqf(s2), s = s2 * is;
     const float R = r * s, z = r * u;
     if (r > E.r_flow_max || R < 1e-3f) return false;
-    const float zh = z * rcpf(E.h_r * R);
+    const float zh = u * is * E.inv_h_r;   // z / (H/R R) = u / (H/R sin theta)
     const float lnvert = -0.5f * zh * zh;
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
+    if constexpr (RADIO == bhd::RADIO_GRID)
+        if (P.turb_on) turb_fetch_f32(P, R, r, u, phid, td, q.tf);   // loads in flight until flow_finish
     const float lr = lnf(r);
-    float dens = 1.f;   // turbulence factor x horizon softening: one logarithm for both
-    if (P.turb_on) {
-        if constexpr (RADIO == bhd::RADIO_SHEARED) dens = sheared_factor_f32(P, r, R, z, phid, td);
-        else dens = turb_factor_f32(P, r, u, phid, td);
-    }
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
     if (!(soft > 0.f)) return false;
-    const float ln_ne = E.ln_n0 - E.n_index * lr + lnvert + lnf(dens * soft);
-    if (ln_ne < -69.07755f) return false;   // n_e < 1e-30
     const float ln_th = E.ln_theta0 - E.theta_index * lr;
-    const float ln_B = 0.5f * (E.ln_bconst + ln_ne - lr);
     // fluid four-velocity (emission.cuh:138-149)
     const float r2 = r * r, a = E.a, a2 = E.a2;
     const float Sigma = r2 + a2 * u * u, Delta = r2 - 2.f * r + a2;
     const float A = (r2 + a2) * (r2 + a2) - a2 * Delta * s2;
-    const float iS = rcpf(Sigma), iA = rcpf(A);
+    const float iSA = rcpf(Sigma * A), iS = A * iSA, iA = Sigma * iSA;   // 1 / Sigma, 1 / A from one reciprocal
     const float g_tt = -(1.f - 2.f * r * iS), g_tph = -2.f * a * r * s2 * iS, g_phph = A * s2 * iS;
     const float omz = 2.f * a * r * iA;
     const float Rk = fmaxf(R, E.r_isco);
     float Om = E.flow_kep * rcpf(Rk * sqrtf_(Rk) + a);
-    const float vmax = sqrtf_(Sigma * Delta * iA) * __builtin_amdgcn_rsqf(g_phph) * 0.999f;
+    // sqrt(Sigma Delta / A) / sqrt(g_phph) = Sigma sqrt(Delta) / (A sin theta)
+    const float vmax = Sigma * iA * is * sqrtf_(Delta) * 0.999f;
     Om = omz + fmaxf(-vmax, fminf(vmax, Om - omz));
     const float norm = -(g_tt + 2.f * g_tph * Om + g_phph * Om * Om);
     if (!(norm > 0.f)) return false;
@@ -369,15 +427,34 @@ __device__ __forceinline__ bool flow_sample(const TraceParams& P, const EmisPara
     const float u_ph = ut * (g_tph + g_phph * Om);
     const float kb = lam - kdotu * u_ph;
     const float bb = g_phph + u_ph * u_ph;
-    const float cosb = kb * rcpf(kdotu * sqrtf_(bb));
-    const float sinb2 = fmaxf(1.f - cosb * cosb, 1e-6f);
-    f.ln_ne = ln_ne;
+    const float cosb2 = kb * kb * rcpf(kdotu * kdotu * bb);   // cos^2 of the pitch angle
+    const float sinb2 = fmaxf(1.f - cosb2, 1e-6f);
     f.ln_th = ln_th;
     f.ln_nu = ln_nu;
-    f.ln_nuc = E.ln_nuc_per_B + ln_B;
     f.ln_sinb = 0.5f * lnf(sinb2);
     f.ln_g = E.ln_nu_obs - ln_nu;
     f.ln_dl = lnf(kdotu * Sigma * (float)hd) + E.ln_rg;   // k.u Sigma h: within FP32 range inside the flow
+    q.lr = lr;
+    q.lnvert = lnvert;
+    q.soft = soft;
+    q.R = R;
+    q.z = z;
+    return true;
+}
+
+template <int RADIO>
+__device__ __forceinline__ bool flow_finish(const TraceParams& P, const EmisParams& E, double rd, double phid,
+                                            double td, FlowSample& f, const FlowPending& q) {
+    float dens = 1.f;   // turbulence factor x horizon softening: one logarithm for both
+    if (P.turb_on) {
+        if constexpr (RADIO == bhd::RADIO_SHEARED) dens = sheared_factor_f32(P, (float)rd, q.R, q.z, phid, td);
+        else dens = turb_blend_f32(q.tf);
+    }
+    const float ln_ne = E.ln_n0 - E.n_index * q.lr + q.lnvert + lnf(dens * q.soft);
+    if (ln_ne < -69.07755f) return false;   // n_e < 1e-30
+    const float ln_B = 0.5f * (E.ln_bconst + ln_ne - q.lr);
+    f.ln_ne = ln_ne;
+    f.ln_nuc = E.ln_nuc_per_B + ln_B;
     return true;
 }
 
@@ -450,7 +527,7 @@ struct RayStateM {
     RayConst c;
     State s;
     double h, acc0, acc1, acc2, acc3, trans;
-    float psi;
+    float psi, s_sin;   // swept angle; sin theta of the current state
     int ncross, accepted, flow_steps, it, fate, fp64_attempts;
     bool done, k1_f32;
 };
@@ -473,6 +550,7 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
     R.acc0 = R.acc1 = R.acc2 = R.acc3 = 0.0;
     R.trans = 1.0;
     R.psi = 0.f;
+    R.s_sin = sqrtf_(fmaxf((float)(1.0 - R.s.y2 * R.s.y2), 0.f));
     R.ncross = R.accepted = R.flow_steps = R.it = R.fate = R.fp64_attempts = 0;
     R.done = P.max_steps <= 0;
 }
@@ -484,10 +562,11 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     State n;
     double hc = R.h;
     if (flow && R.s.y0 < M.r_cut) {
-        double sn = sqrt(fmax(1.0 - R.s.y2 * R.s.y2, 0.0));
-        const double frac = R.it >= M.relax_after ? P.step_frac * M.relax_cap : P.step_frac;
-        double cap = frac * fmin(R.s.y0 * rcpd(fabs(R.s.y1) + 1e-30), fmax(sn, 0.05) * rcpd(fabs(R.s.y3) + 1e-30));
-        hc = fmin(hc, cap);
+        // the transfer step cap (trace.cuh:87-90) in FP32: a bound on the step, not part of its accuracy
+        const float frac = (float)(R.it >= M.relax_after ? P.step_frac * M.relax_cap : P.step_frac);
+        const float cap = frac * fminf((float)R.s.y0 * rcpf(fabsf((float)R.s.y1) + 1e-30f),
+                                       fmaxf(R.s_sin, 0.05f) * rcpf(fabsf((float)R.s.y3) + 1e-30f));
+        hc = fmin(hc, (double)cap);
     }
     double err;
     if (R.s.y0 >= M.r_f32) {
@@ -515,24 +594,30 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         return;
     }
     ++R.accepted;
+    // the flow sample's density-free part first: its turbulence loads are in flight during the ring
+    // angle below
+    const bool in_flow = flow && n.y0 < M.r_cut;
+    FlowSample f;
+    FlowPending q;
+    bool emits = false;
+    if (in_flow) emits = flow_begin<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f, q);
+    // sin theta of the new state (FP64 1 - u^2, then FP32): the ring angle and the next step cap
+    const float s1 = sqrtf_(fmaxf((float)(1.0 - n.y2 * n.y2), 0.f));
     if (RADIO) {
-        // angle between the radial directions of the previous and the new accepted state, in the
-        // frame rotated to the previous azimuth: p = (s0, 0, u0), q = (s1 cos dphi, s1 sin dphi, u1),
-        // with dphi taken in FP64 (no stored direction, no large-angle FP32 azimuth)
-        const float u0 = (float)R.s.y2, u1 = (float)n.y2;
-        const float s0 = sqrtf_(fmaxf((float)(1.0 - R.s.y2 * R.s.y2), 0.f));
-        const float s1 = sqrtf_(fmaxf((float)(1.0 - n.y2 * n.y2), 0.f));
-        float sd, cd;
-        __sincosf((float)(n.y4 - R.s.y4), &sd, &cd);
-        const float a1 = u0 * s1;
-        const float cx = a1 * sd, cy = a1 * cd - s0 * u1, cz = s0 * s1 * sd;
-        R.psi += atan2_pos(sqrtf_(cx * cx + cy * cy + cz * cz), s0 * s1 * cd + u0 * u1);
+        // angle between the radial directions p = (s0 cos phi0, s0 sin phi0, u0) of the previous and
+        // q of the new accepted state, from their chord without cancellation:
+        // |q - p|^2 = (u1 - u0)^2 + (s1 - s0)^2 + 4 s0 s1 sin^2(dphi / 2), angle = 2 asin(|q - p| / 2),
+        // with dphi taken in FP64 (no large-angle FP32 azimuth)
+        const float s0 = R.s_sin, du = (float)n.y2 - (float)R.s.y2, ds = s1 - s0;
+        const float hs = __sinf(0.5f * (float)(n.y4 - R.s.y4));
+        const float chord = sqrtf_(du * du + ds * ds + 4.f * s0 * s1 * hs * hs);
+        R.psi += 2.f * asin_pos(fminf(0.5f * chord, 1.f));
     }
+    R.s_sin = s1;
     if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
-    if (flow && n.y0 < M.r_cut) {
+    if (in_flow) {
         ++R.flow_steps;
-        FlowSample f;
-        if (flow_sample<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f)) {
+        if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q)) {
             float ln_j, ln_a, ln_bnu, fj, fa;
             if (M.use_surrogate &&
                 surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
@@ -563,7 +648,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     }
     const double r0 = R.s.y0;
     R.s = n;
-    bhd::project_radial(P, R.c, R.s);
+    project_radial_m(P, R.c, R.s);
     double fac = fmin(5.0, fmax(0.2, 0.9 * pow_m02_d(err)));
     // Outside the scene-tolerance region the step follows the radial scale: there the Mino-time
     // step that meets the tolerance goes as 1/r (dr/dlambda ~ r^2), and the error-only controller

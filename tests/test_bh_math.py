@@ -134,3 +134,20 @@ def test_identities(oracle_bhm):
     assert np.max(np.abs(at - t)) < 2e-15
     u = rng.uniform(-1.0, 1.0, 200_000)
     assert np.max(np.abs(np.cos(B("acos", u)) - u)) < 1e-15
+
+
+def test_pow_m02_correctly_rounded(oracle_bhm):
+    """The step controller's x^-0.2 (with -0.2 the double nearest -1/5) is correctly rounded on
+    [1e-300, 1e300]: equal to the 200-bit value rounded to nearest on 30,000 log-uniform arguments
+    and on arguments next to exact powers (where the result is close to a round number)."""
+    mpmath = pytest.importorskip("mpmath")
+    rng = np.random.default_rng(11)
+    x = np.exp(rng.uniform(np.log(1e-300), np.log(1e300), 30_000))
+    near = np.array([2.0 ** (5 * k) for k in range(-199, 200)])
+    x = np.concatenate([x, near, np.nextafter(near, 0), np.nextafter(near, np.inf)])
+    got = oracle_bhm.math_eval("pow_m02", x)
+    mpmath.mp.prec = 200
+    p = mpmath.mpf(-0.2)   # the double -0.2, exactly
+    want = np.array([float(mpmath.mpf(float(v)) ** p) for v in x])
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, f"{len(bad)} misrounded, first x={x[bad[:3]]!r}: {got[bad[:3]]!r} vs {want[bad[:3]]!r}"
