@@ -9,20 +9,25 @@
    for bit; accuracy against glibc is <= 3 ulp (pinned by tests/test_bh_math.py over 10^6 inputs per
    function), and x^-0.2 is correctly rounded.
 
-   Methods (all textbook, coefficients are exact series terms rounded to double):
-   - exp, expm1: Cody-Waite reduction x = k ln2 + r with a 32-bit ln2 head, |r| <= ln2/2, Taylor
-     series of degree 13 (exp) / 14 (expm1) in Horner form; expm1 recombines 2^k expm1(r) + (2^k-1).
+   Methods (textbook reductions; the polynomials are near-minimax fits of this project, least squares
+   on Chebyshev nodes with Lawson reweighting in 120-bit arithmetic, coefficients rounded to double;
+   the fit error is below 2^-58 relative in every case). Polynomials are evaluated in Horner form
+   with explicit fused multiply-adds: an FMA is correctly rounded on both sides (v_fma_f64 on the
+   device, the C library's fma on the host), so the results stay bit-identical while each Horner
+   step costs one instruction instead of a separate multiply and add.
+   - exp, expm1: Cody-Waite reduction x = k ln2 + r with a 32-bit ln2 head, |r| <= ln2/2;
+     e^r - 1 = r + r^2 q(r) with q of degree 10; expm1 recombines 2^k expm1(r) + (2^k-1).
    - log: x = m 2^e, m in [sqrt(1/2), sqrt(2)], f = m-1, s = f/(2+f), log(1+f) = f - s (f - R) with
-     R = sum_{n>=1} 2 s^{2n}/(2n+1) (the atanh series), to n = 12.
-   - cbrt: m in [1,2) and the exponent split by 3; Newton iteration w <- w (4 - m w^3)/3 for
-     m^{-1/3} from a cubic least-squares seed, then cbrt(m) = m w^2.
-   - sincos: Cody-Waite reduction by pi/2 with a three-part constant (33+33+53 bits), Taylor series
-     on |r| <= pi/4 (sin to r^17, cos to r^18).
-   - atan2/acos: reciprocal and pi/6 (Hart) reductions to |t| <= tan(pi/12), odd Taylor series to
-     t^29; acos(x) = atan2(sqrt((1-x)(1+x)), x), asin(x) = atan2(x, sqrt((1-x)(1+x))).
+     R = 2 atanh(s)/s - 2 = z P(z), z = s^2, P of degree 6.
+   - cbrt: m in [1,2) and the exponent split by 3; two Newton steps w <- w (4 - m w^3)/3 for
+     m^{-1/3} from a cubic least-squares seed (5e-4), cbrt(m) = m w^2, then one Newton polish.
+   - sincos: Cody-Waite reduction by pi/2 with a three-part constant (33+33+53 bits);
+     sin r = r + r z S(z) (S of degree 5), cos r = 1 + z C(z) (C of degree 6) on |r| <= pi/4.
+   - atan2/acos: reciprocal and pi/6 (Hart) reductions to |t| <= tan(pi/12),
+     atan t = t + t z A(z) with A of degree 8; acos(x) = atan2(sqrt((1-x)(1+x)), x),
+     asin(x) = atan2(x, sqrt((1-x)(1+x))).
    - x^-0.2 (the step controller): exp(-0.2 log x) refined by one Newton step on x y^5 = 1 with an
-     FMA double-double residual; correctly rounded on [1e-300, 1e300]. FMA is exact on both sides
-     (v_fma_f64 on the device, the C library's fma on the host), so this stays bit-identical.
+     FMA double-double residual; correctly rounded on [1e-300, 1e300].
    Subnormal results of exp are flushed to zero (deterministically on both sides). */
 #pragma once
 #include <stdint.h>
@@ -59,29 +64,36 @@ constexpr double SQRT2 = 0x1.6a09e667f3bcdp+0;
 constexpr double CBRT2 = 0x1.428a2f98d728bp+0;
 constexpr double CBRT4 = 0x1.965fea53d6e3dp+0;
 
-// 1/n!
-constexpr double F2 = 0x1.0000000000000p-1, F3 = 0x1.5555555555555p-3, F4 = 0x1.5555555555555p-5,
-                 F5 = 0x1.1111111111111p-7, F6 = 0x1.6c16c16c16c17p-10, F7 = 0x1.a01a01a01a01ap-13,
-                 F8 = 0x1.a01a01a01a01ap-16, F9 = 0x1.71de3a556c734p-19, F10 = 0x1.27e4fb7789f5cp-22,
-                 F11 = 0x1.ae64567f544e4p-26, F12 = 0x1.1eed8eff8d898p-29, F13 = 0x1.6124613a86d09p-33,
-                 F14 = 0x1.93974a8c07c9dp-37, F15 = 0x1.ae7f3e733b81fp-41, F16 = 0x1.ae7f3e733b81fp-45,
-                 F17 = 0x1.952c77030ad4ap-49, F18 = 0x1.6827863b97d97p-53;
+BHM_HD double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// e^r - 1 - r for |r| <= ln2/2, as r^2 * q(r)
+// One Horner step p x + c with c a constant coefficient. gfx950 has no 64-bit literal operand, and
+// the compiler turns fma(p, x, literal) into v_fmac_f64 with the coefficient as the tied addend,
+// copied into a VGPR pair first (two v_mov_b32 per step). Here the coefficient sits in a scalar
+// register pair (one VOP3 operand may be scalar) and the step is one v_fma_f64: one vector
+// instruction instead of three. Same correctly rounded FMA as the host's.
+BHM_HD double horner_(double p, double x, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(p), "v"(x), "s"(c));
+    return d;
+#else
+    return __builtin_fma(p, x, c);
+#endif
+}
+
+// (e^r - 1 - r) / r^2 for |r| <= ln2/2, degree 10 (fit error 2.6e-18)
 BHM_HD double expm1_core_q(double r) {
-    double p = F14;
-    p = p * r + F13;
-    p = p * r + F12;
-    p = p * r + F11;
-    p = p * r + F10;
-    p = p * r + F9;
-    p = p * r + F8;
-    p = p * r + F7;
-    p = p * r + F6;
-    p = p * r + F5;
-    p = p * r + F4;
-    p = p * r + F3;
-    p = p * r + F2;
+    double p = 0x1.1f8b3e103c1fep-29;
+    p = horner_(p, r, 0x1.af4dde3ddccc9p-26);
+    p = horner_(p, r, 0x1.27e4ccd670e98p-22);
+    p = horner_(p, r, 0x1.71de0236f22adp-19);
+    p = horner_(p, r, 0x1.a01a01acf7424p-16);
+    p = horner_(p, r, 0x1.a01a01abe651ep-13);
+    p = horner_(p, r, 0x1.6c16c16c15206p-10);
+    p = horner_(p, r, 0x1.11111111100dfp-7);
+    p = horner_(p, r, 0x1.5555555555558p-5);
+    p = horner_(p, r, 0x1.5555555555557p-3);
+    p = fma_(p, r, 0.5);
     return p;
 }
 
@@ -89,10 +101,10 @@ BHM_HD double exp(double x) {
     if (isnan_(x)) return x;
     if (x > 709.782712893384) return __builtin_inf();
     if (x < -708.3964185322641) return 0.0;   // result would be subnormal: flushed
-    double kd = floor_(x * INV_LN2 + 0.5);
+    double kd = floor_(fma_(x, INV_LN2, 0.5));
     int k = (int)kd;
-    double r = (x - kd * LN2_HI) - kd * LN2_LO;
-    double e = 1.0 + (r + (r * r) * expm1_core_q(r));
+    double r = fma_(-kd, LN2_LO, fma_(-kd, LN2_HI, x));
+    double e = 1.0 + fma_(r * r, expm1_core_q(r), r);
     if (k > 1023) return e * 2.0 * pow2i(k - 1);
     if (k < -1021) return (e * pow2i(k + 54)) * 0x1p-54;
     return e * pow2i(k);
@@ -102,18 +114,18 @@ BHM_HD double expm1(double x) {
     if (isnan_(x)) return x;
     if (x > 709.782712893384) return __builtin_inf();
     if (x < -38.0) return -1.0;   // e^x < 2^-54
-    if (fabs_(x) <= 0.34657359027997264) return x + (x * x) * expm1_core_q(x);
-    double kd = floor_(x * INV_LN2 + 0.5);
+    if (fabs_(x) <= 0.34657359027997264) return fma_(x * x, expm1_core_q(x), x);
+    double kd = floor_(fma_(x, INV_LN2, 0.5));
     int k = (int)kd;
-    double r = (x - kd * LN2_HI) - kd * LN2_LO;
-    double em = r + (r * r) * expm1_core_q(r);
+    double r = fma_(-kd, LN2_LO, fma_(-kd, LN2_HI, x));
+    double em = fma_(r * r, expm1_core_q(r), r);
     if (k > 56) {
         double e = em + 1.0;
         if (k > 1023) return e * 2.0 * pow2i(k - 1);
         return e * pow2i(k);
     }
     double t = pow2i(k);
-    return t * em + (t - 1.0);
+    return fma_(t, em, t - 1.0);
 }
 
 BHM_HD double log(double x) {
@@ -135,22 +147,18 @@ BHM_HD double log(double x) {
     double f = m - 1.0;
     double s = f / (2.0 + f);
     double z = s * s;
-    double R = 0x1.47ae147ae147bp-4;   // 2/25
-    R = R * z + 0x1.642c8590b2164p-4;  // 2/23
-    R = R * z + 0x1.8618618618618p-4;  // 2/21
-    R = R * z + 0x1.af286bca1af28p-4;  // 2/19
-    R = R * z + 0x1.e1e1e1e1e1e1ep-4;  // 2/17
-    R = R * z + 0x1.1111111111111p-3;  // 2/15
-    R = R * z + 0x1.3b13b13b13b14p-3;  // 2/13
-    R = R * z + 0x1.745d1745d1746p-3;  // 2/11
-    R = R * z + 0x1.c71c71c71c71cp-3;  // 2/9
-    R = R * z + 0x1.2492492492492p-2;  // 2/7
-    R = R * z + 0x1.999999999999ap-2;  // 2/5
-    R = R * z + 0x1.5555555555555p-1;  // 2/3
-    R = R * z;
-    double lm = f - s * (f - R);
+    // R = 2 atanh(s) / s - 2 = z P(z), P of degree 6 (fit error of R 3.6e-18)
+    double P = 0x1.2f056aef6994ap-3;
+    P = horner_(P, z, 0x1.39a1b9ea2f5d1p-3);
+    P = horner_(P, z, 0x1.74663ef13705bp-3);
+    P = horner_(P, z, 0x1.c71c52092cd21p-3);
+    P = horner_(P, z, 0x1.24924941f169ep-2);
+    P = horner_(P, z, 0x1.999999997fdb2p-2);
+    P = horner_(P, z, 0x1.5555555555592p-1);
+    double R = z * P;
+    double lm = fma_(-s, f - R, f);
     double ed = (double)e;
-    return ed * LN2_HI + (lm + ed * LN2_LO);
+    return fma_(ed, LN2_HI, fma_(ed, LN2_LO, lm));
 }
 
 // x^-0.2 for x > 0 (the step-size controller's factor, with -0.2 the double nearest -1/5), correctly
@@ -193,7 +201,7 @@ BHM_HD double cbrt(double x) {
     w = w * m + 0x1.3cab785e5a648p-2;
     w = w * m + -0x1.977b7db5d7b17p-1;
     w = w * m + 0x1.88cc68a4a6346p+0;
-    for (int i = 0; i < 3; ++i) w = w * (4.0 - m * (w * w * w)) * (1.0 / 3.0);
+    for (int i = 0; i < 2; ++i) w = w * fma_(-m, w * w * w, 4.0) * (1.0 / 3.0);   // 5e-4 -> 6e-13
     double y = m * (w * w);
     double t = m;
     if (rr == 1) {
@@ -203,7 +211,7 @@ BHM_HD double cbrt(double x) {
         y = y * CBRT4;
         t = m * 4.0;
     }
-    y = y + (t / (y * y) - y) * (1.0 / 3.0);   // one Newton polish of y^3 = t
+    y = fma_(t / (y * y) - y, 1.0 / 3.0, y);   // one Newton polish of y^3 = t
     int k = q + adj;
     return sgn * (y * pow2i(k));
 }
@@ -215,29 +223,27 @@ BHM_HD void sincos(double x, double* sp, double* cp) {
         *cp = __builtin_nan("");
         return;
     }
-    double kd = floor_(x * TWO_OVER_PI + 0.5);
-    double r = ((x - kd * PIO2_1) - kd * PIO2_2) - kd * PIO2_3;
+    double kd = floor_(fma_(x, TWO_OVER_PI, 0.5));
+    double r = fma_(-kd, PIO2_3, fma_(-kd, PIO2_2, fma_(-kd, PIO2_1, x)));
     int q = (int)(kd - 4.0 * floor_(kd * 0.25));
     double z = r * r;
-    // sin r = r + r z (-1/3! + z (1/5! - ... + z/17!)), cos r = 1 + z (-1/2 + z (1/4! - ... - z/18!))
-    double ps = F17;
-    ps = ps * z - F15;
-    ps = ps * z + F13;
-    ps = ps * z - F11;
-    ps = ps * z + F9;
-    ps = ps * z - F7;
-    ps = ps * z + F5;
-    ps = ps * z - F3;
-    double s = r + (r * z) * ps;
-    double pc = -F18;
-    pc = pc * z + F16;
-    pc = pc * z - F14;
-    pc = pc * z + F12;
-    pc = pc * z - F10;
-    pc = pc * z + F8;
-    pc = pc * z - F6;
-    pc = pc * z + F4;
-    double c = 1.0 + z * (pc * z - F2);
+    // sin r = r + r z S(z) (S of degree 5), cos r = 1 + z C(z) (C of degree 6); fit errors
+    // 9.0e-18 relative to r and 1.6e-18 absolute on |r| <= pi/4
+    double ps = 0x1.5d6496562cae0p-33;
+    ps = horner_(ps, z, -0x1.ae5db3c31d54fp-26);
+    ps = horner_(ps, z, 0x1.71de34747192dp-19);
+    ps = horner_(ps, z, -0x1.a01a019ab3813p-13);
+    ps = horner_(ps, z, 0x1.111111110eb14p-7);
+    ps = horner_(ps, z, -0x1.555555555553dp-3);
+    double s = fma_(r * z, ps, r);
+    double pc = -0x1.8f803897f4c42p-37;
+    pc = horner_(pc, z, 0x1.1ee9844185923p-29);
+    pc = horner_(pc, z, -0x1.27e4f752d538ep-22);
+    pc = horner_(pc, z, 0x1.a01a019b6e90ep-16);
+    pc = horner_(pc, z, -0x1.6c16c16c13f0cp-10);
+    pc = horner_(pc, z, 0x1.555555555553cp-5);
+    pc = fma_(pc, z, -0.5);
+    double c = fma_(z, pc, 1.0);
     switch (q) {
         case 0: *sp = s; *cp = c; break;
         case 1: *sp = c; *cp = -s; break;
@@ -253,21 +259,17 @@ BHM_HD double atan_pos(double t) {
     bool red = t > TAN_PI_12;
     if (red) t = (t * SQRT3 - 1.0) / (t + SQRT3);
     double z = t * t;
-    double p = 0x1.1a7b9611a7b96p-5;     // 1/29
-    p = p * z - 0x1.2f684bda12f68p-5;    // 1/27
-    p = p * z + 0x1.47ae147ae147bp-5;    // 1/25
-    p = p * z - 0x1.642c8590b2164p-5;    // 1/23
-    p = p * z + 0x1.8618618618618p-5;    // 1/21
-    p = p * z - 0x1.af286bca1af28p-5;    // 1/19
-    p = p * z + 0x1.e1e1e1e1e1e1ep-5;    // 1/17
-    p = p * z - 0x1.1111111111111p-4;    // 1/15
-    p = p * z + 0x1.3b13b13b13b14p-4;    // 1/13
-    p = p * z - 0x1.745d1745d1746p-4;    // 1/11
-    p = p * z + 0x1.c71c71c71c71cp-4;    // 1/9
-    p = p * z - 0x1.2492492492492p-3;    // 1/7
-    p = p * z + 0x1.999999999999ap-3;    // 1/5
-    p = p * z - 0x1.5555555555555p-2;    // 1/3
-    double a = t + (t * z) * p;
+    // atan t = t + t z A(z), A of degree 8 (fit error 1.6e-18 relative to t)
+    double p = -0x1.3ae4ad1dc9d8ap-5;
+    p = horner_(p, z, 0x1.d15d3acf0adbfp-5);
+    p = horner_(p, z, -0x1.1065d886df6fdp-4);
+    p = horner_(p, z, 0x1.3b0b3bb362accp-4);
+    p = horner_(p, z, -0x1.745cd4f68b712p-4);
+    p = horner_(p, z, 0x1.c71c708c231dbp-4);
+    p = horner_(p, z, -0x1.24924922ea4b0p-3);
+    p = horner_(p, z, 0x1.9999999997695p-3);
+    p = horner_(p, z, -0x1.555555555554dp-2);
+    double a = fma_(t * z, p, t);
     if (red) a = PI_6 + a;
     if (inv) a = PI_2 - a;
     return a;
