@@ -814,6 +814,10 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
     M.tol_relax = o.tol_relax > 0 ? o.tol_relax : std::max(S.tol, DEFAULT_TOL_RELAX);
     M.relax_cap = o.relax_cap > 0 ? o.relax_cap : DEFAULT_RELAX_CAP;
     if (M.relax_after <= 0) M.relax_after = 0x7fffffff;
+    M.inv_tol = 1.0 / S.tol;
+    M.inv_tol_mid = 1.0 / M.tol_mid;
+    M.inv_tol_far = 1.0 / M.tol_far;
+    M.inv_tol_relax = 1.0 / M.tol_relax;
     bhm32::EmisParams& E = M.E;
     E.ln_n0 = (float)std::log(S.n0);
     E.n_index = (float)S.n_index;
@@ -848,6 +852,8 @@ int check_opts(bh_ctx* c, const bh_trace_opts& o) {
     if (o.emissivity == BH_EMIS_MLP && !c->stab_ready)
         return fail(c, BH_ERR_UNSUPPORTED, "no baked surrogate: call bh_mlp_train (or bh_mlp_set) and bh_mlp_bake first");
     if (o.engine != BH_ENGINE_AUTO && o.engine != BH_ENGINE_FRAME) return fail(c, BH_ERR_ARG, "engine must be 0 or 1");
+    if (o.precision == BH_PREC_MIXED && !(c->S.tol > 0.0))
+        return fail(c, BH_ERR_ARG, "the mixed-precision tracer needs a scene tolerance > 0");
     return BH_OK;
 }
 

@@ -1,7 +1,11 @@
 /* peak.hip - MI355X peaks the tracer's rooflines are priced against, measured on the machine:
    vector FP64 FMA rate, vector FP32 FMA rate as this code is compiled (packed FP32 instructions
    disabled), HBM copy bandwidth, and the shader clock while the FMA kernel runs (shader cycle
-   counter against the constant 100 MHz clock). Every loop is bounded and every access in range.
+   counter against the constant 100 MHz clock). Then the issue cost of single instructions for ONE
+   wave alone on the chip (what bounds a frame's last, longest ray): shader cycles per instruction
+   for 8 independent chains (the issue interval) and for one dependent chain (the latency), for the
+   FP64 FMA, the FP64 reciprocal, the FP32 exp2 and the FP32 FMA. Every loop is bounded and every
+   access in range.
 
    usage: bhrt_peak [iterations]        (prints one line per figure) */
 #include <hip/hip_runtime.h>
@@ -82,6 +86,70 @@ static void fma_rate(const char* name, int cus, int iters) {
     CK(hipEventDestroy(e1));
 }
 
+// One wave: CHAINS instructions OP per iteration spread over NCH independent chains (CHAINS: the
+// issue interval; 1: the dependent latency); shader cycles of the loop in clk[0]
+enum Op { OP_FMA64, OP_RCP64, OP_EXP32, OP_FMA32 };
+
+template <Op OP>
+__device__ __forceinline__ void op_step(double* a, float* f, int k) {
+    if constexpr (OP == OP_FMA64) a[k] = __builtin_fma(a[k], 0.999999, 1e-7);
+    if constexpr (OP == OP_RCP64) a[k] = __builtin_amdgcn_rcp(a[k]);             // x, 1/x, x, ...
+    if constexpr (OP == OP_EXP32) f[k] = __builtin_amdgcn_exp2f(-f[k]);          // stays in (0, 1]
+    if constexpr (OP == OP_FMA32) f[k] = __builtin_fmaf(f[k], 0.999999f, 1e-7f);
+}
+
+template <Op OP, int NCH>
+__global__ __launch_bounds__(64) void k_issue(const double* __restrict__ in, double* __restrict__ out, int iters,
+                                              unsigned long long* __restrict__ clk) {
+    const int i = threadIdx.x;
+    double a[NCH];
+    float f[NCH];
+    for (int k = 0; k < NCH; ++k) {
+        a[k] = in[k] + 1e-3 * i;
+        f[k] = (float)a[k];
+    }
+    const unsigned long long c0 = clock64();
+    for (int it = 0; it < iters; ++it) {   // CHAINS instructions per iteration either way
+#pragma unroll
+        for (int k = 0; k < CHAINS; ++k) op_step<OP>(a, f, k % NCH);
+    }
+    const unsigned long long c1 = clock64();
+    double s = 0;
+    for (int k = 0; k < NCH; ++k) s += a[k] + (double)f[k];
+    out[i] = s;
+    if (i == 0) clk[0] = c1 - c0;
+}
+
+template <Op OP, int NCH>
+static double issue_cycles(int iters) {
+    double h[CHAINS];
+    for (int k = 0; k < CHAINS; ++k) h[k] = 0.5 + 0.0625 * k;
+    double *din, *dout;
+    unsigned long long* dclk;
+    CK(hipMalloc(&din, sizeof h));
+    CK(hipMalloc(&dout, 64 * sizeof(double)));
+    CK(hipMalloc(&dclk, sizeof(unsigned long long)));
+    CK(hipMemcpy(din, h, sizeof h, hipMemcpyHostToDevice));
+    for (int r = 0; r < 2; ++r) {   // the second launch is the one read
+        hipLaunchKernelGGL((k_issue<OP, NCH>), dim3(1), dim3(64), 0, 0, din, dout, iters, dclk);
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+    }
+    unsigned long long clk = 0;
+    CK(hipMemcpy(&clk, dclk, sizeof clk, hipMemcpyDeviceToHost));
+    CK(hipFree(din));
+    CK(hipFree(dout));
+    CK(hipFree(dclk));
+    return (double)clk / ((double)iters * CHAINS);
+}
+
+template <Op OP>
+static void issue_line(const char* name, int iters) {
+    std::printf("one wave, %s: %.2f cycles per instruction over %d independent chains, %.2f in one dependent "
+                "chain\n",
+                name, issue_cycles<OP, CHAINS>(iters), CHAINS, issue_cycles<OP, 1>(iters));
+}
+
 static void copy_rate(int cus) {
     const size_t bytes = (size_t)4 << 30, n = bytes / sizeof(float4);
     float4 *src, *dst;
@@ -122,5 +190,9 @@ int main(int argc, char** argv) {
     fma_rate<double>("FP64", p.multiProcessorCount, iters);
     fma_rate<float>("FP32 (no packed FP32)", p.multiProcessorCount, iters);
     copy_rate(p.multiProcessorCount);
+    issue_line<OP_FMA64>("v_fma_f64", iters);
+    issue_line<OP_RCP64>("v_rcp_f64", iters);
+    issue_line<OP_EXP32>("v_exp_f32", iters);
+    issue_line<OP_FMA32>("v_fma_f32", iters);
     return 0;
 }

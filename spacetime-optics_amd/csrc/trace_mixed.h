@@ -144,8 +144,7 @@ __device__ __forceinline__ Rhs<R> rhs(const RConst<R>& c, R r, R u) {
 /* One Dormand-Prince 5(4) step with stage arithmetic in R (float: far field, double: near field),
    the 5th-order state accumulated in FP64 and the scaled error norm in FP64. */
 template <typename R>
-__device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, double h, double tol, double tol4,
-                                       double inv_tol_t,
+__device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, double h, double inv_tol, double inv_tol_t,
                                        State& o) {
     using namespace bhd;
     const R hr = (R)h;
@@ -206,16 +205,18 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     o.k5 = (double)k.d5;
     E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
     E5 += R(e7) * k.d5;
-    // scaled max-norm over r, r', u, u', phi (kerr.hpp:145-155) in FP64, with Newton-refined FP64
-    // reciprocals of the scales in place of the divisions
-    const double sc0 = tol * (1e-6 + fmax(fabs(s.y0), fabs(o.y0)));
-    const double sc1 = tol * (1e-6 + fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0));
-    const double sc3 = tol * fmax(1.0, fmax(fabs(s.y3), fabs(o.y3)));
-    double m = fabs(h * (double)E0) * rcpd(sc0);
-    m = fmax(m, fabs(h * (double)E1) * rcpd(sc1));
-    m = fmax(m, fabs(h * (double)E2) * rcpd(tol));
-    m = fmax(m, fabs(h * (double)E3) * rcpd(sc3));
-    m = fmax(m, fabs(h * (double)E4) * rcpd(tol4));
+    // scaled max-norm over r, r', u, u', phi (kerr.hpp:145-155) in FP64. Each of its scales is tol
+    // times a factor a_i (a = 1 for u, 4 for phi), so the norm is max_i(|h E_i| / a_i) / tol: three
+    // Newton-refined FP64 reciprocals for the state-dependent factors, and 1 / tol precomputed
+    const double a0 = 1e-6 + fmax(fabs(s.y0), fabs(o.y0));
+    const double a1 = 1e-6 + fmax(fmax(fabs(s.y1), fabs(o.y1)), s.y0 * s.y0);
+    const double a3 = fmax(1.0, fmax(fabs(s.y3), fabs(o.y3)));
+    double m = fabs(h * (double)E0) * rcpd(a0);
+    m = fmax(m, fabs(h * (double)E1) * rcpd(a1));
+    m = fmax(m, fabs(h * (double)E2));
+    m = fmax(m, fabs(h * (double)E3) * rcpd(a3));
+    m = fmax(m, fabs(h * (double)E4) * 0.25);
+    m *= inv_tol;
     // coordinate time (not in the reference's norm): an absolute bound per step where the turbulence
     // depends on the emission time (inv_tol_t = 1 / bound, 0 when it does not)
     m = fmax(m, fabs(h * (double)E5) * inv_tol_t);
@@ -516,6 +517,8 @@ struct MixedParams {
     double tol_relax; // emission-region tolerance of a ray after relax_after attempts
     double relax_cap; // and the factor on its transfer step cap
     int relax_after;
+    // reciprocals of the scene tolerance, tol_mid, tol_far and tol_relax (the error norm's 1 / tol)
+    double inv_tol, inv_tol_mid, inv_tol_far, inv_tol_relax;
     EmisParams E;
     int use_surrogate;
     float ln_e2_over_c, ln_e2_over_mec;   // ln(e^2 / c), ln(e^2 / (m_e c))
@@ -571,7 +574,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     double err;
     if (R.s.y0 >= M.r_f32) {
         const RConst<float> cf = make_const<float>(P, R.c);
-        err = dp54<float>(cf, R.s, hc, M.tol_far, M.tol_far * 4.0, M.inv_tol_t, n);
+        err = dp54<float>(cf, R.s, hc, M.inv_tol_far, M.inv_tol_t, n);
     } else {
         const RConst<double> cd = make_const<double>(P, R.c);
         if (R.k1_f32) {   // entering the FP64 region: FP64 derivative at the current state
@@ -580,9 +583,9 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         }
         // inside the emission region the scene tolerance, relaxed for a ray that has already taken
         // relax_after attempts (an orbit near the photon sphere)
-        const double tol = R.s.y0 < M.r_tol ? (R.it >= M.relax_after ? M.tol_relax : P.tol)
-                                            : (R.s.y0 < M.r_far ? M.tol_mid : M.tol_far);
-        err = dp54<double>(cd, R.s, hc, tol, tol * 4.0, M.inv_tol_t, n);
+        const double inv_tol = R.s.y0 < M.r_tol ? (R.it >= M.relax_after ? M.inv_tol_relax : M.inv_tol)
+                                                : (R.s.y0 < M.r_far ? M.inv_tol_mid : M.inv_tol_far);
+        err = dp54<double>(cd, R.s, hc, inv_tol, M.inv_tol_t, n);
         ++R.fp64_attempts;
     }
     R.k1_f32 = R.s.y0 >= M.r_f32;
