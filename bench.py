@@ -185,6 +185,8 @@ def main():
     ap.add_argument("--no-mlp-train", action="store_true", help="skip MLP modes")
     ap.add_argument("--mlp-save", default="", help="write the trained surrogate (parameters, normalisation) to this .npz")
     ap.add_argument("--mlp-load", default="", help="load the surrogate from this .npz instead of training (profiling)")
+    ap.add_argument("--mlp-epochs", type=int, default=0,
+                    help="training epochs of the surrogate (0: the library's default; fewer for a short kernel trace)")
     ap.add_argument("--copy-calib", type=int, default=0, help="MiB of a device copy before the modes (profiling)")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive-sampling render")
     ap.add_argument("--batch-frames", type=int, default=4,
@@ -220,7 +222,7 @@ def main():
             rep = {"loss_first": None, "loss_last": None}
             setup["mlp_loaded"] = args.mlp_load
         else:
-            rep = tracer.mlp_train()
+            rep = tracer.mlp_train(epochs=args.mlp_epochs)
         if args.mlp_save:
             p, nrm = tracer.mlp_get()
             np.savez(args.mlp_save, params=p, norm=nrm)

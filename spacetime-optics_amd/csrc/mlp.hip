@@ -248,6 +248,11 @@ __global__ __launch_bounds__(256) void k_mlp_sample(uint64_t seed, int n, const 
     ws[i] = 1.f + hot_weight / (1.f + expf(-(float)(fj + 12.0)));
 }
 
+// 16-sample tiles per gradient workgroup (one wave, 72 KB of LDS): the default batch of 8192 samples
+// spreads over 256 workgroups, one per CU. Measured per training step: 4 tiles 104 us (half the CUs
+// idle), 2 tiles 83 us, 1 tile 92 us (twice the weight loads and partial gradients to reduce).
+constexpr int GRAD_TILES = 2;
+
 /* One training step's gradient over `count` samples starting at `first` (indices mod n): each
    workgroup (one wave) takes `tiles` consecutive 16-sample tiles and writes its partial gradient
    (8706 floats) and partial loss. Loss = sum_i w_i |y_i - t_i|^2 / count on normalised outputs. */
@@ -353,34 +358,64 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, No
     if (lane == 0) loss_partial[blockIdx.x] = loss;
 }
 
-// grad[p] = sum over workgroups of partial[w][p] (fixed order); loss likewise
-__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int nw,
-                                                   const float* __restrict__ loss_partial, float* __restrict__ grad,
-                                                   float* __restrict__ loss) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// grad[p] = sum over workgroups of partial[w][p]; loss likewise. A workgroup takes 64 parameters; its
+// four waves each sum a quarter of the workgroups' partials (w = quarter, quarter + 4, ...) in eight
+// running sums (eight independent loads in flight), and the quarters are added through LDS. Every
+// sum is taken in a fixed order, so the result does not depend on timing.
+// With P set, the same lane then takes the Adam step of its parameter (bias-corrected moments).
+struct AdamStep {
+    float* P;   // nullptr: gradient only
+    float* m;
+    float* v;
+    float lr, b1, b2, eps, bc1, bc2;
+};
+
+constexpr int RED_PARAMS = 64, RED_SPLIT = 4, RED_BLOCK = RED_PARAMS * RED_SPLIT;
+__global__ __launch_bounds__(RED_BLOCK) void k_mlp_reduce(const float* __restrict__ partial, int nw,
+                                                         const float* __restrict__ loss_partial,
+                                                         float* __restrict__ grad, float* __restrict__ loss,
+                                                         AdamStep A) {
+    __shared__ float quarter_sum[RED_SPLIT][RED_PARAMS];
+    const int lp = threadIdx.x % RED_PARAMS, qt = threadIdx.x / RED_PARAMS;
+    const int p = blockIdx.x * RED_PARAMS + lp;
+    float total = 0.f;
     if (p < NPARAM) {
-        float s = 0.f;
-        for (int w = 0; w < nw; ++w) s += partial[(size_t)w * NPARAM + p];
-        grad[p] = s;
+        constexpr size_t STEP = (size_t)RED_SPLIT * NPARAM;   // between this quarter's rows
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, sr = 0.f;
+        const float* q = partial + (size_t)qt * NPARAM + p;
+        int w = qt;
+        for (; w + 7 * RED_SPLIT < nw; w += 8 * RED_SPLIT, q += 8 * STEP) {
+            s0 += q[0];
+            s1 += q[STEP];
+            s2 += q[2 * STEP];
+            s3 += q[3 * STEP];
+            s4 += q[4 * STEP];
+            s5 += q[5 * STEP];
+            s6 += q[6 * STEP];
+            s7 += q[7 * STEP];
+        }
+        for (; w < nw; w += RED_SPLIT, q += STEP) sr += q[0];
+        total = (((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7))) + sr;
     }
-    if (p == 0) {
-        float s = 0.f;
-        for (int w = 0; w < nw; ++w) s += loss_partial[w];
-        *loss = s;
+    quarter_sum[qt][lp] = total;
+    __syncthreads();
+    if (qt == 0 && p < NPARAM) {
+        const float gr = (quarter_sum[0][lp] + quarter_sum[1][lp]) + (quarter_sum[2][lp] + quarter_sum[3][lp]);
+        grad[p] = gr;
+        if (A.P) {
+            const float mm = A.b1 * A.m[p] + (1.f - A.b1) * gr, vv = A.b2 * A.v[p] + (1.f - A.b2) * gr * gr;
+            A.m[p] = mm;
+            A.v[p] = vv;
+            A.P[p] -= A.lr * (mm / A.bc1) / (sqrtf(vv / A.bc2) + A.eps);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        float sl = 0.f;
+        for (int v = 0; v < nw; ++v) sl += loss_partial[v];
+        *loss = sl;
     }
 }
 
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, const float* __restrict__ grad,
-                                             float* __restrict__ m, float* __restrict__ v, float lr, float b1,
-                                             float b2, float eps, float bc1, float bc2) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= NPARAM) return;
-    const float gr = grad[p];
-    const float mm = b1 * m[p] + (1.f - b1) * gr, vv = b2 * v[p] + (1.f - b2) * gr * gr;
-    m[p] = mm;
-    v[p] = vv;
-    P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
-}
 
 /* Inference: n samples of raw inputs (u0, Theta, theta_B) or, with lattice != 0, the nodes of the
    (n0 x n1 x n2) lattice over (u0, log10 Theta, theta_B in [lo2, hi2]); writes (ln f_j, ln f_a). */
@@ -586,7 +621,7 @@ int bh_mlp_train(bh_ctx* c, const bh_mlp_train_cfg* cfg_in, bh_mlp_report* rep) 
     const int n = cfg.n_samples;
     float *xs = nullptr, *ts = nullptr, *ws = nullptr, *partial = nullptr, *lpart = nullptr, *grad = nullptr,
           *loss = nullptr;
-    const int tiles = 4;
+    const int tiles = GRAD_TILES;
     const int nwg = (cfg.batch / TM + tiles - 1) / tiles;
     auto cleanup = [&]() {
         for (float* p : {xs, ts, ws, partial, lpart, grad, loss})
@@ -643,13 +678,12 @@ int bh_mlp_train(bh_ctx* c, const bh_mlp_train_cfg* cfg_in, bh_mlp_report* rep) 
         const int first = (int)(((long long)step * cfg.batch) % n);
         hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, c->st, c->d_mlp, N, xs, ts, ws, n, first, cfg.batch,
                            tiles, partial, lpart);
-        hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, c->st, partial, nwg, lpart, grad,
-                           loss);
         const double prog = (double)step / std::max(total - 1, 1);
         const float lr = (float)(cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1.0 + std::cos(3.141592653589793 * prog)));
         const float bc1 = (float)(1.0 - std::pow(0.9, step + 1)), bc2 = (float)(1.0 - std::pow(0.999, step + 1));
-        hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, c->st, c->d_mlp, grad, c->d_adam_m,
-                           c->d_adam_v, lr, 0.9f, 0.999f, 1e-8f, bc1, bc2);
+        const AdamStep A{c->d_mlp, c->d_adam_m, c->d_adam_v, lr, 0.9f, 0.999f, 1e-8f, bc1, bc2};
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + RED_PARAMS - 1) / RED_PARAMS), dim3(RED_BLOCK), 0, c->st,
+                           partial, nwg, lpart, grad, loss, A);
         if (step == 0 || step == total - 1) {
             float l = 0;
             if (hipMemcpyAsync(&l, loss, sizeof(float), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
@@ -773,7 +807,7 @@ int bh_mlp_grad(bh_ctx* c, const float* params, const float* norm, const float* 
     if (!c || !params || !norm || !xn || !tn || !w || !grad_out || n <= 0 || n % TM != 0) return BH_ERR_ARG;
     MCK(c, hipSetDevice(c->device));
     const Norm N = norm_from_floats(norm);
-    const int tiles = 4, nwg = (n / TM + tiles - 1) / tiles;
+    const int tiles = GRAD_TILES, nwg = (n / TM + tiles - 1) / tiles;
     float *dp = nullptr, *dx = nullptr, *dt = nullptr, *dw = nullptr, *part = nullptr, *lp = nullptr, *g = nullptr,
           *l = nullptr;
     int rc = BH_OK;
@@ -791,7 +825,8 @@ int bh_mlp_grad(bh_ctx* c, const float* params, const float* norm, const float* 
         rc = BH_ERR_HIP;
     } else {
         hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, c->st, dp, N, dx, dt, dw, n, 0, n, tiles, part, lp);
-        hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + 255) / 256), dim3(256), 0, c->st, part, nwg, lp, g, l);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + RED_PARAMS - 1) / RED_PARAMS), dim3(RED_BLOCK), 0, c->st, part,
+                           nwg, lp, g, l, AdamStep{});
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(grad_out, g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
             (loss_out && hipMemcpyAsync(loss_out, l, sizeof(float), hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
