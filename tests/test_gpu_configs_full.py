@@ -1,4 +1,4 @@
-"""The surrogate frame and the batched animation at their stated sizes.
+"""The surrogate frame, the batched animation and the photon-ring zoom at their stated sizes.
 
 - 4K frame (shots/c4_mlp_4k.ini): 3840 x 2160 rays of the 230 GHz flow, mixed precision + trained
   surrogate, against the FP64 frame: flux within 0.1 % and per-pixel median within 1 %; one FP64
@@ -7,7 +7,9 @@
   spins 0 / 0.5 / 0.94 / 0.998, 32 time slices of a 64^3 turbulence grid read at each sample's
   emission time, mixed precision + surrogate, in one persistent launch: bit-identical to rendering the
   frames one by one; the 512^2 frames within the accuracy bars of their FP64 frames; two FP64 tiles
-  (different spins and sizes) bit for bit against the oracle."""
+  (different spins and sizes) bit for bit against the oracle.
+- Photon-ring zoom (shots/c3_zoom_2048.ini): 2048^2 rays at tolerance 1e-12, FP64 on the persistent
+  engine bit-identical to the frame kernel, the mixed tracer capturing the same rays."""
 import numpy as np
 import pytest
 
@@ -112,5 +114,35 @@ def test_c5_full_size_one_launch(oracle_bhm):
             _, rays, _ = oracle_bhm.trace_pixels(OS, oracle_frame(F), px, py, spp=1, jitter=False)
             assert_rays_equal(gpu, rays, f"c5 frame {i} tile")
             assert (gpu["I"][..., 0] > 0).any()
+    finally:
+        t.close()
+
+
+def test_c3_full_size_zoom():
+    """The photon-ring zoom at its stated size: 2048^2 rays in the 0.0005 M window on the critical
+    curve, tolerance 1e-12 (every ray orbits; step counts spread over several times the mean). The
+    FP64 frame on the persistent engine equals the frame kernel's bit for bit, counters included, and
+    no ray reaches the step limit; the mixed tracer captures the same rays on all but 0.1 % of the
+    pixels (FP64 near the photon sphere)."""
+    t, S, F, O, _ = bh.Tracer.from_ini(shot("c3_zoom_2048.ini"), normalize=False)
+    try:
+        assert (F.W, F.H) == (2048, 2048) and S.tol == 1e-12
+        img, st = t.render(F, t.opts(precision=bh.PREC_FP64))
+        n = F.W * F.H
+        print(f"c3 2048^2 FP64, persistent: {st.ms_kernel:.1f} ms ({n / st.ms_kernel * 1e-3:.1f} Mrays/s), "
+              f"{st.step_attempts / n:.0f} attempts per ray", flush=True)
+        ref, sf = t.render(F, t.opts(precision=bh.PREC_FP64, engine=bh.ENGINE_FRAME))
+        print(f"c3 2048^2 FP64, frame kernel: {sf.ms_kernel:.1f} ms", flush=True)
+        assert np.array_equal(img.view(np.int32), ref.view(np.int32))
+        assert (st.step_attempts, st.steps_accepted, st.step_limited) == \
+               (sf.step_attempts, sf.steps_accepted, sf.step_limited)
+        assert st.step_limited == 0
+        mix, sm = t.render(F, t.opts(precision=bh.PREC_MIXED))
+        cap64, capm = np.all(img[..., :3] == 0, axis=-1), np.all(mix[..., :3] == 0, axis=-1)
+        agree = float((cap64 == capm).mean())
+        print(f"c3 2048^2 mixed: {sm.ms_kernel:.1f} ms ({n / sm.ms_kernel * 1e-3:.1f} Mrays/s), "
+              f"{sm.step_attempts / n:.0f} attempts per ray; captured-pixel agreement {agree:.5f} "
+              f"(captured {cap64.mean():.3f})", flush=True)
+        assert agree > 0.999 and 0.01 < cap64.mean() < 0.99
     finally:
         t.close()
