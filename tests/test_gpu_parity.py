@@ -251,6 +251,18 @@ def test_error_paths():
         t.trace_tile(F, 0, 0, 0, 4)
     with pytest.raises(bh.BhError):
         t.render(F, bh.Tracer.opts(precision=7))
+    big = bh.Frame.from_buffer_copy(F)
+    big.W = 70000
+    with pytest.raises(bh.BhError, match="bad frame size"):
+        t.render(big)
+    # the mixed tracer scales its error norm by 1 / tol: a scene tolerance of 0 is refused
+    S0 = bh.Scene.from_buffer_copy(cfg["S"])
+    S0.tol = 0.0
+    t.update_scene(S0)
+    with pytest.raises(bh.BhError, match="tolerance"):
+        t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED))
+    t.update_scene(cfg["S"])
+    assert t.render(F, bh.Tracer.opts(precision=bh.PREC_MIXED))[1].samples == F.W * F.H
     # radio without the K2 table is refused, not traced
     S = cfg["S"]
     S2 = bh.Scene.from_buffer_copy(S)
