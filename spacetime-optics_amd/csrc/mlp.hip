@@ -34,19 +34,25 @@ constexpr int TM = 16;           // samples per tile
 constexpr int LDA = NH + 4;      // padded row of an activation tile (floats)
 constexpr int LDW = NH + 8;      // padded row of a BF16 weight matrix (halves)
 
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-    unsigned int u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 
+// two floats to packed BF16 (a in the low half), rounded to nearest even: one v_cvt_pk_bf16_f32
+__device__ __forceinline__ unsigned int bf16x2(float a, float b) {
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector((fl2){a, b}, bf2));
+}
+// the low / high BF16 of a packed pair as a float
+__device__ __forceinline__ float bf16_lo(unsigned int p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned int p) { return __uint_as_float(p & 0xffff0000u); }
+
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), four values at once
 __device__ __forceinline__ void split4(const float v[4], s4& hi, s4& lo) {
-    for (int e = 0; e < 4; ++e) {
-        unsigned short h = bf16_rne(v[e]);
-        hi[e] = (short)h;
-        lo[e] = (short)bf16_rne(v[e] - bf16_f(h));
-    }
+    const unsigned int h01 = bf16x2(v[0], v[1]), h23 = bf16x2(v[2], v[3]);
+    const unsigned int l01 = bf16x2(v[0] - bf16_lo(h01), v[1] - bf16_hi(h01));
+    const unsigned int l23 = bf16x2(v[2] - bf16_lo(h23), v[3] - bf16_hi(h23));
+    hi = __builtin_bit_cast(s4, (u2){h01, h23});
+    lo = __builtin_bit_cast(s4, (u2){l01, l23});
 }
 
 __device__ __forceinline__ f4 mfma3(s4 ah, s4 al, s4 bh, s4 bl, f4 c) {
@@ -61,7 +67,7 @@ __device__ __forceinline__ float dsilu(float z) {
     return s * (1.f + z * (1.f - s));
 }
 
-struct WeightsLDS {
+struct alignas(16) WeightsLDS {   // 16-byte aligned: load_weights stores BF16 pairs as 32-bit words
     unsigned short w2h[NH * LDW], w2l[NH * LDW], w3h[NH * LDW], w3l[NH * LDW];
 };
 struct TileLDS {
@@ -71,16 +77,17 @@ struct TileLDS {
     float y[TM][2], t[TM][2], w[TM];
 };
 
-// split the FP32 master W2, W3 (row-major [k][j]) into BF16 head/tail arrays in LDS
+// split the FP32 master W2, W3 (row-major [k][j]) into BF16 head/tail arrays in LDS, two adjacent
+// j at a time (LDW is even, so each pair is one aligned 32-bit store)
 __device__ void load_weights(const float* __restrict__ P, WeightsLDS& W) {
-    for (int e = threadIdx.x; e < NH * NH; e += 64) {
-        int k = e / NH, j = e % NH;
-        float a = P[OFF_W2 + e], b = P[OFF_W3 + e];
-        unsigned short ah = bf16_rne(a), bh = bf16_rne(b);
-        W.w2h[k * LDW + j] = ah;
-        W.w2l[k * LDW + j] = bf16_rne(a - bf16_f(ah));
-        W.w3h[k * LDW + j] = bh;
-        W.w3l[k * LDW + j] = bf16_rne(b - bf16_f(bh));
+    for (int e = 2 * threadIdx.x; e < NH * NH; e += 2 * 64) {
+        const int k = e / NH, j = e % NH;
+        const float a0 = P[OFF_W2 + e], a1 = P[OFF_W2 + e + 1], b0 = P[OFF_W3 + e], b1 = P[OFF_W3 + e + 1];
+        const unsigned int ah = bf16x2(a0, a1), bh = bf16x2(b0, b1);
+        *(unsigned int*)&W.w2h[k * LDW + j] = ah;
+        *(unsigned int*)&W.w2l[k * LDW + j] = bf16x2(a0 - bf16_lo(ah), a1 - bf16_hi(ah));
+        *(unsigned int*)&W.w3h[k * LDW + j] = bh;
+        *(unsigned int*)&W.w3l[k * LDW + j] = bf16x2(b0 - bf16_lo(bh), b1 - bf16_hi(bh));
     }
 }
 
