@@ -80,7 +80,7 @@ struct TileLDS {
 // split the FP32 master W2, W3 (row-major [k][j]) into BF16 head/tail arrays in LDS, two adjacent
 // j at a time (LDW is even, so each pair is one aligned 32-bit store)
 __device__ void load_weights(const float* __restrict__ P, WeightsLDS& W) {
-    for (int e = 2 * threadIdx.x; e < NH * NH; e += 2 * 64) {
+    for (int e = 2 * threadIdx.x; e < NH * NH; e += 2 * blockDim.x) {
         const int k = e / NH, j = e % NH;
         const float a0 = P[OFF_W2 + e], a1 = P[OFF_W2 + e + 1], b0 = P[OFF_W3 + e], b1 = P[OFF_W3 + e + 1];
         const unsigned int ah = bf16x2(a0, a1), bh = bf16x2(b0, b1);
@@ -255,20 +255,48 @@ __global__ __launch_bounds__(256) void k_mlp_sample(uint64_t seed, int n, const 
     ws[i] = 1.f + hot_weight / (1.f + expf(-(float)(fj + 12.0)));
 }
 
-// 16-sample tiles per gradient workgroup (one wave, 72 KB of LDS): the default batch of 8192 samples
-// spreads over 256 workgroups, one per CU. Measured per training step: 4 tiles 104 us (half the CUs
-// idle), 2 tiles 83 us, 1 tile 92 us (twice the weight loads and partial gradients to reduce).
-constexpr int GRAD_TILES = 2;
+// Gradient workgroups: GRAD_WAVES waves share one copy of the split weights in LDS, and each wave
+// takes GRAD_TILES 16-sample tiles with its own activation buffers (36 KB of weights + 35 KB per
+// wave). The default batch of 8192 samples is then 256 workgroups, one per CU with two busy SIMDs.
+// Measured per training step with one wave per workgroup: 4 tiles 104 us (half the CUs idle),
+// 2 tiles 83 us, 1 tile 92 us (twice the weight loads and partial gradients to reduce).
+constexpr int GRAD_WAVES = 2, GRAD_TILES = 1;
+static_assert(GRAD_WAVES == 2, "k_mlp_grad combines the partial gradients of exactly two waves");
 
 /* One training step's gradient over `count` samples starting at `first` (indices mod n): each
    workgroup (one wave) takes `tiles` consecutive 16-sample tiles and writes its partial gradient
    (8706 floats) and partial loss. Loss = sum_i w_i |y_i - t_i|^2 / count on normalised outputs. */
-__global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, Norm N, const float* __restrict__ xs,
+// The partial gradient held by one lane, visited in the parameter layout: f(index, value)
+template <typename F>
+__device__ __forceinline__ void for_partial(int lane, const float gW1[3], float gb1, float gb2, float gb3,
+                                            const float gW4[2], float gb4, const f4 gW2[4][4], const f4 gW3[4][4],
+                                            F f) {
+    const int r = lane & 15, g = lane >> 4;
+    for (int k = 0; k < 3; ++k) f(OFF_W1 + k * NH + lane, gW1[k]);
+    f(OFF_B1 + lane, gb1);
+    f(OFF_B2 + lane, gb2);
+    f(OFF_B3 + lane, gb3);
+    f(OFF_W4 + lane * NOUT, gW4[0]);
+    f(OFF_W4 + lane * NOUT + 1, gW4[1]);
+    if (lane < 2) f(OFF_B4 + lane, gb4);
+    for (int mb = 0; mb < 4; ++mb)
+        for (int jb = 0; jb < 4; ++jb)
+            for (int e = 0; e < 4; ++e) {
+                const int m = 16 * mb + 4 * g + e, j = 16 * jb + r;
+                f(OFF_W2 + m * NH + j, gW2[mb][jb][e]);
+                f(OFF_W3 + m * NH + j, gW3[mb][jb][e]);
+            }
+}
+
+__global__ __launch_bounds__(64 * GRAD_WAVES) void k_mlp_grad(const float* __restrict__ P, Norm N, const float* __restrict__ xs,
                                                  const float* __restrict__ ts, const float* __restrict__ ws, int n,
                                                  int first, int count, int tiles, float* __restrict__ partial,
                                                  float* __restrict__ loss_partial) {
     __shared__ WeightsLDS W;
-    __shared__ TileLDS T;
+    __shared__ TileLDS TT[GRAD_WAVES];
+    __shared__ float wave_loss[GRAD_WAVES];
+    const int wave = threadIdx.x >> 6;
+    TileLDS& T = TT[wave];
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     load_weights(P, W);
     __syncthreads();
@@ -278,7 +306,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, No
     float gW1[3] = {0.f, 0.f, 0.f}, gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, gW4[2] = {0.f, 0.f}, gb4 = 0.f, loss = 0.f;
     const float inv_count = 1.f / (float)count;
     for (int tl = 0; tl < tiles; ++tl) {
-        const int base = (blockIdx.x * tiles + tl) * TM;   // sample offset inside the batch
+        const int base = ((blockIdx.x * GRAD_WAVES + wave) * tiles + tl) * TM;   // sample offset in the batch
         if (lane < TM) {
             const int s = lane;
             const bool in = base + s < count;
@@ -344,31 +372,26 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ P, No
         }
         __syncthreads();
     }
-    // write this workgroup's partial gradient in the parameter layout
-    float* out = partial + (size_t)blockIdx.x * NPARAM;
-    for (int k = 0; k < 3; ++k) out[OFF_W1 + k * NH + lane] = gW1[k];
-    out[OFF_B1 + lane] = gb1;
-    out[OFF_B2 + lane] = gb2;
-    out[OFF_B3 + lane] = gb3;
-    out[OFF_W4 + lane * NOUT] = gW4[0];
-    out[OFF_W4 + lane * NOUT + 1] = gW4[1];
-    if (lane < 2) out[OFF_B4 + lane] = gb4;
-    for (int mb = 0; mb < 4; ++mb)
-        for (int jb = 0; jb < 4; ++jb)
-            for (int e = 0; e < 4; ++e) {
-                const int m = 16 * mb + 4 * g + e, j = 16 * jb + r;
-                out[OFF_W2 + m * NH + j] = gW2[mb][jb][e];
-                out[OFF_W3 + m * NH + j] = gW3[mb][jb][e];
-            }
-    // partial loss: lanes 0..31 hold pieces
+    // the workgroup's partial gradient: wave 1 leaves its sums in its own (no longer used) tile
+    // buffer, wave 0 adds them to its own and writes the result, always in that order
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_down(loss, off, 64);
-    if (lane == 0) loss_partial[blockIdx.x] = loss;
+    if (lane == 0) wave_loss[wave] = loss;
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(&TT[1]);
+    static_assert(sizeof(TileLDS) >= sizeof(float) * NPARAM, "a tile buffer holds a partial gradient");
+    if (wave == 1) for_partial(lane, gW1, gb1, gb2, gb3, gW4, gb4, gW2, gW3, [&](int i, float v) { red[i] = v; });
+    __syncthreads();
+    if (wave == 0) {
+        float* out = partial + (size_t)blockIdx.x * NPARAM;
+        for_partial(lane, gW1, gb1, gb2, gb3, gW4, gb4, gW2, gW3, [&](int i, float v) { out[i] = v + red[i]; });
+        if (lane == 0) loss_partial[blockIdx.x] = wave_loss[0] + wave_loss[1];
+    }
 }
 
 // grad[p] = sum over workgroups of partial[w][p]; loss likewise. A workgroup takes 64 parameters; its
-// four waves each sum a quarter of the workgroups' partials (w = quarter, quarter + 4, ...) in eight
-// running sums (eight independent loads in flight), and the quarters are added through LDS. Every
-// sum is taken in a fixed order, so the result does not depend on timing.
+// eight waves each sum an eighth of the workgroups' partials (w = q, q + 8, ...) in eight running
+// sums (eight independent loads in flight), and the eighths are added through LDS. Every sum is
+// taken in a fixed order, so the result does not depend on timing.
 // With P set, the same lane then takes the Adam step of its parameter (bias-corrected moments).
 struct AdamStep {
     float* P;   // nullptr: gradient only
@@ -377,17 +400,18 @@ struct AdamStep {
     float lr, b1, b2, eps, bc1, bc2;
 };
 
-constexpr int RED_PARAMS = 64, RED_SPLIT = 4, RED_BLOCK = RED_PARAMS * RED_SPLIT;
+constexpr int RED_PARAMS = 64, RED_SPLIT = 8, RED_BLOCK = RED_PARAMS * RED_SPLIT;
 __global__ __launch_bounds__(RED_BLOCK) void k_mlp_reduce(const float* __restrict__ partial, int nw,
                                                          const float* __restrict__ loss_partial,
                                                          float* __restrict__ grad, float* __restrict__ loss,
                                                          AdamStep A) {
-    __shared__ float quarter_sum[RED_SPLIT][RED_PARAMS];
+    __shared__ float part_sum[RED_SPLIT][RED_PARAMS];
+    static_assert(RED_SPLIT == 8, "the final sum below adds eight parts");
     const int lp = threadIdx.x % RED_PARAMS, qt = threadIdx.x / RED_PARAMS;
     const int p = blockIdx.x * RED_PARAMS + lp;
     float total = 0.f;
     if (p < NPARAM) {
-        constexpr size_t STEP = (size_t)RED_SPLIT * NPARAM;   // between this quarter's rows
+        constexpr size_t STEP = (size_t)RED_SPLIT * NPARAM;   // between this group's rows
         float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f, sr = 0.f;
         const float* q = partial + (size_t)qt * NPARAM + p;
         int w = qt;
@@ -404,10 +428,11 @@ __global__ __launch_bounds__(RED_BLOCK) void k_mlp_reduce(const float* __restric
         for (; w < nw; w += RED_SPLIT, q += STEP) sr += q[0];
         total = (((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7))) + sr;
     }
-    quarter_sum[qt][lp] = total;
+    part_sum[qt][lp] = total;
     __syncthreads();
     if (qt == 0 && p < NPARAM) {
-        const float gr = (quarter_sum[0][lp] + quarter_sum[1][lp]) + (quarter_sum[2][lp] + quarter_sum[3][lp]);
+        const float(&q)[RED_SPLIT][RED_PARAMS] = part_sum;
+        const float gr = ((q[0][lp] + q[1][lp]) + (q[2][lp] + q[3][lp])) + ((q[4][lp] + q[5][lp]) + (q[6][lp] + q[7][lp]));
         grad[p] = gr;
         if (A.P) {
             const float mm = A.b1 * A.m[p] + (1.f - A.b1) * gr, vv = A.b2 * A.v[p] + (1.f - A.b2) * gr * gr;
@@ -416,10 +441,11 @@ __global__ __launch_bounds__(RED_BLOCK) void k_mlp_reduce(const float* __restric
             A.P[p] -= A.lr * (mm / A.bc1) / (sqrtf(vv / A.bc2) + A.eps);
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x < 64) {   // the loss: one wave, strided sums and a fixed tree
         float sl = 0.f;
-        for (int v = 0; v < nw; ++v) sl += loss_partial[v];
-        *loss = sl;
+        for (int v = threadIdx.x; v < nw; v += 64) sl += loss_partial[v];
+        for (int off = 32; off > 0; off >>= 1) sl += __shfl_down(sl, off, 64);
+        if (threadIdx.x == 0) *loss = sl;
     }
 }
 
@@ -629,7 +655,7 @@ int bh_mlp_train(bh_ctx* c, const bh_mlp_train_cfg* cfg_in, bh_mlp_report* rep) 
     float *xs = nullptr, *ts = nullptr, *ws = nullptr, *partial = nullptr, *lpart = nullptr, *grad = nullptr,
           *loss = nullptr;
     const int tiles = GRAD_TILES;
-    const int nwg = (cfg.batch / TM + tiles - 1) / tiles;
+    const int nwg = (cfg.batch / TM + tiles * GRAD_WAVES - 1) / (tiles * GRAD_WAVES);
     auto cleanup = [&]() {
         for (float* p : {xs, ts, ws, partial, lpart, grad, loss})
             if (p) (void)hipFree(p);
@@ -683,7 +709,8 @@ int bh_mlp_train(bh_ctx* c, const bh_mlp_train_cfg* cfg_in, bh_mlp_report* rep) 
     float first_loss = -1.f, last_loss = -1.f;
     for (int step = 0; step < total; ++step) {
         const int first = (int)(((long long)step * cfg.batch) % n);
-        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, c->st, c->d_mlp, N, xs, ts, ws, n, first, cfg.batch,
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64 * GRAD_WAVES), 0, c->st, c->d_mlp, N, xs, ts, ws, n, first,
+                           cfg.batch,
                            tiles, partial, lpart);
         const double prog = (double)step / std::max(total - 1, 1);
         const float lr = (float)(cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1.0 + std::cos(3.141592653589793 * prog)));
@@ -814,7 +841,7 @@ int bh_mlp_grad(bh_ctx* c, const float* params, const float* norm, const float* 
     if (!c || !params || !norm || !xn || !tn || !w || !grad_out || n <= 0 || n % TM != 0) return BH_ERR_ARG;
     MCK(c, hipSetDevice(c->device));
     const Norm N = norm_from_floats(norm);
-    const int tiles = GRAD_TILES, nwg = (n / TM + tiles - 1) / tiles;
+    const int tiles = GRAD_TILES, nwg = (n / TM + tiles * GRAD_WAVES - 1) / (tiles * GRAD_WAVES);
     float *dp = nullptr, *dx = nullptr, *dt = nullptr, *dw = nullptr, *part = nullptr, *lp = nullptr, *g = nullptr,
           *l = nullptr;
     int rc = BH_OK;
@@ -831,7 +858,8 @@ int bh_mlp_grad(bh_ctx* c, const float* params, const float* norm, const float* 
         hipMemcpy(dw, w, sizeof(float) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
         rc = BH_ERR_HIP;
     } else {
-        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64), 0, c->st, dp, N, dx, dt, dw, n, 0, n, tiles, part, lp);
+        hipLaunchKernelGGL(k_mlp_grad, dim3(nwg), dim3(64 * GRAD_WAVES), 0, c->st, dp, N, dx, dt, dw, n, 0, n, tiles,
+                           part, lp);
         hipLaunchKernelGGL(k_mlp_reduce, dim3((NPARAM + RED_PARAMS - 1) / RED_PARAMS), dim3(RED_BLOCK), 0, c->st, part,
                            nwg, lp, g, l, AdamStep{});
         if (hipGetLastError() != hipSuccess ||
