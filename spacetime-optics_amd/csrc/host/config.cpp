@@ -2,7 +2,6 @@
 
 #include <cstdlib>
 #include <fstream>
-#include <sstream>
 
 namespace bh {
 
@@ -67,38 +66,52 @@ bool Config::load(const std::string& path, std::string* err) {
     return true;
 }
 
+const std::string* Config::find(const std::string& key) const {
+    const auto it = kv_.find(key);
+    return it == kv_.end() ? nullptr : &it->second;
+}
+
 std::string Config::str(const std::string& key, const std::string& def) const {
-    auto it = kv_.find(key);
-    return it == kv_.end() ? def : it->second;
+    const std::string* v = find(key);
+    return v ? *v : def;
 }
 
+// The number a value starts with (strtod's prefix rule: trailing text is ignored); the default for
+// a missing key or a value that does not start with a number (the empty value included).
 double Config::num(const std::string& key, double def) const {
-    auto it = kv_.find(key);
-    if (it == kv_.end() || it->second.empty()) return def;
-    const char* s = it->second.c_str();
-    char* end = nullptr;
-    double v = std::strtod(s, &end);
-    return end == s ? def : v;
+    const std::string* v = find(key);
+    if (!v) return def;
+    const char* begin = v->c_str();
+    char* stop = nullptr;
+    const double x = std::strtod(begin, &stop);
+    return stop == begin ? def : x;
 }
 
+// The words bhrt's shot files use for switches (exact, lower case); anything else keeps the default
 bool Config::flag(const std::string& key, bool def) const {
-    auto it = kv_.find(key);
-    if (it == kv_.end()) return def;
-    const std::string& v = it->second;
-    if (v == "1" || v == "true" || v == "yes" || v == "on") return true;
-    if (v == "0" || v == "false" || v == "no" || v == "off") return false;
+    static const char* const on_words[] = {"1", "true", "yes", "on"};
+    static const char* const off_words[] = {"0", "false", "no", "off"};
+    const std::string* v = find(key);
+    if (!v) return def;
+    for (const char* w : on_words)
+        if (*v == w) return true;
+    for (const char* w : off_words)
+        if (*v == w) return false;
     return def;
 }
 
+// Items between separators, trimmed; empty items are dropped
 std::vector<std::string> Config::list(const std::string& key, char sep) const {
-    std::vector<std::string> out;
-    std::stringstream ss(str(key));
-    std::string item;
-    while (std::getline(ss, item, sep)) {
-        item = trim(item);
-        if (!item.empty()) out.push_back(item);
+    std::vector<std::string> items;
+    const std::string s = str(key);
+    for (size_t from = 0; from <= s.size();) {
+        size_t to = s.find(sep, from);
+        if (to == std::string::npos) to = s.size();
+        std::string item = trim(s.substr(from, to - from));
+        if (!item.empty()) items.push_back(std::move(item));
+        from = to + 1;
     }
-    return out;
+    return items;
 }
 
 bool Config::apply_set(const std::string& kv) {

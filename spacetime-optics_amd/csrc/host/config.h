@@ -27,6 +27,7 @@ public:
     bool apply_set(const std::string& kv);
 
 private:
+    const std::string* find(const std::string& key) const;   // nullptr when the key is not set
     std::map<std::string, std::string> kv_;
     std::string dir_ = ".";
 };
