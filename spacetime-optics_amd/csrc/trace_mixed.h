@@ -266,13 +266,12 @@ __device__ __forceinline__ float table_lerp_f32(const float* __restrict__ t, int
 // ln sin(theta_B) (Leung et al. 2011 with the K2 table, Kirchhoff alpha = j / B_nu). With
 // y = X^(1/3): sqrt(X) + 2^(11/12) X^(1/6) = X^(1/6) (y + 2^(11/12)), so ln j needs one exponential
 // and one logarithm; X is capped at e^240 (where j = exp(-y) is already 0 in FP32) so y stays finite.
-__device__ __forceinline__ void synchrotron_exact(const TraceParams& P, const EmisParams& E, float ln_ne,
-                                                  float ln_th, float ln_nu, float ln_nuc, float ln_sinb,
-                                                  float* ln_j, float* ln_a, float* ln_bnu) {
+__device__ __forceinline__ void synchrotron_exact(const EmisParams& E, float ln_ne, float ln_th, float lnk2,
+                                                  float ln_nu, float ln_nuc, float ln_sinb, float* ln_j,
+                                                  float* ln_a, float* ln_bnu) {
     const float ln_nus = -1.5040774f + ln_nuc + 2.f * ln_th + ln_sinb;   // ln(2/9) + ...
     const float lnX = fminf(ln_nu - ln_nus, 240.f);
     const float y = expf_(lnX * (1.f / 3.f));
-    const float lnk2 = table_lerp_f32(P.k2, P.k2_n, P.k2_l0, E.k2_inv_dl, ln_th);
     *ln_j = ln_ne + E.ln_jconst + ln_nus - lnk2 + lnX * (1.f / 3.f) + 2.f * lnf(y + 1.887748625363387f) - y;
     // B_nu with x = h nu / (m_e c^2 Theta): ln(2 h nu^3 / c^2) - ln(expm1(x)); ln x is known
     const float ln_x = E.ln_h_over_mec2 + ln_nu - ln_th;
@@ -382,6 +381,7 @@ __device__ __forceinline__ float sheared_factor_f32(const TraceParams& P, float 
 // sample emits and absorbs nothing; the early-outs are those of emission.cuh:99-175.
 struct FlowSample {
     float ln_ne, ln_th, ln_nu, ln_nuc, ln_sinb, ln_g, ln_dl;
+    float ln_k2;   // ln K2(1 / Theta_e) from the table: looked up early, its loads overlap the rest
 };
 struct FlowPending {
     float lr, lnvert, soft, R, z;
@@ -407,6 +407,7 @@ __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParam
     const float soft = 1.f - expf_(-dr * dr * 4.f);
     if (!(soft > 0.f)) return false;
     const float ln_th = E.ln_theta0 - E.theta_index * lr;
+    f.ln_k2 = table_lerp_f32(P.k2, P.k2_n, P.k2_l0, E.k2_inv_dl, ln_th);
     // fluid four-velocity (emission.cuh:138-149)
     const float r2 = r * r, a = E.a, a2 = E.a2;
     const float Sigma = r2 + a2 * u * u, Delta = r2 - 2.f * r + a2;
@@ -629,7 +630,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
                 ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
                 ln_bnu = ln_j - ln_a;   // both within the table's floors: no cancellation
             } else {
-                synchrotron_exact(P, M.E, f.ln_ne, f.ln_th, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a, &ln_bnu);
+                synchrotron_exact(M.E, f.ln_ne, f.ln_th, f.ln_k2, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a, &ln_bnu);
             }
             double dI, dtau;
             float etau;
