@@ -104,9 +104,10 @@ typedef struct bh_trace_opts {
     int emissivity;   /* BH_EMIS_* (radio only) */
     int jitter;       /* 0: pixel-centre samples; 1: R2 + PCG jitter as render.cu:70-85 */
     int spp;          /* samples per pixel; 0 -> frame->spp */
-    double tol_far;   /* mixed: tolerance of the FP32 far-field steps (0 -> vacuum: max(1e-6, tol);
-                         radio: 1e-4 * (pix * r) / 0.04 M, the pixel's size at the hole, clamped to
-                         [max(1e-6, tol), 1e-4]) */
+    double tol_far;   /* mixed: tolerance of the far-field steps (0 -> vacuum: max(1e-6, tol);
+                         radio: 1e-4 * (pix * r) / 0.04 M, the pixel's size at the hole, at most 1e-4,
+                         times sin(theta_cam) / sin(15 deg) for a camera within 15 degrees of the spin
+                         axis, and at least max(1e-6, tol)) */
     double r_near;    /* mixed, vacuum: FP64 below this radius (0 -> max(3 r_+, r_ph) + 1) */
     double r_emit;    /* mixed, radio: the emission region is r < max(r_emit, r_near): scene tolerance,
                          emission, absorption and the transfer step cap there; tol_mid from there to
