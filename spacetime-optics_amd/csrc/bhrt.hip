@@ -813,6 +813,11 @@ bhm32::MixedParams make_mixed(bh_ctx* c, const bh_scene& S, const bh_frame* f, c
                                  c->turb_model == BH_TURB_SHEARED);
     const double tol_t = o.tol_t > 0 ? o.tol_t : (o.tol_t == 0 && time_dependent ? DEFAULT_TOL_T : 0.0);
     M.inv_tol_t = tol_t > 0 ? 1.0 / tol_t : 0.0;
+    // the FP64 stages integrate t only where the emission reads it (time slices, the sheared model,
+    // a t bound); launch_trace_into sets it for ray records as well
+    M.need_t = (S.mode == BH_MODE_RADIO && S.flow_on &&
+                ((c->turb_model == BH_TURB_GRID && c->d_turb && c->turb_nt > 1) ||
+                 c->turb_model == BH_TURB_SHEARED)) || tol_t > 0;
     // long-ray relaxation of the emission-region tolerance
     M.relax_after = o.relax_after > 0 ? o.relax_after : (o.relax_after == 0 ? DEFAULT_RELAX_AFTER : 0x7fffffff);
     M.tol_relax = o.tol_relax > 0 ? o.tol_relax : std::max(S.tol, DEFAULT_TOL_RELAX);
@@ -876,6 +881,7 @@ int launch_trace_into(bh_ctx* c, const bh_frame* f, const bh_trace_opts& o, int 
                       float4* px, bh_ray_out* rays_out, BlockOut* blk, BlockOut* tot_dev, bool may_alloc,
                       bh_stats* st) {
     bhm32::MixedParams M = make_mixed(c, c->S, f, o);
+    if (rays) M.need_t = 1;   // the records carry t
     if (o.emissivity == BH_EMIS_MLP) {
         M.use_surrogate = 1;
         M.S = c->stab;

@@ -125,8 +125,9 @@ struct RConst {
     R a, a2, lam, eta, K, lam2, alam, s2min;
 };
 
-// blackholes/src/kerr.hpp:103-116 with reciprocals in place of the three divisions
-template <typename R>
+// blackholes/src/kerr.hpp:103-116 with reciprocals in place of the three divisions; without TIME
+// the coordinate-time derivative d5 is left at zero
+template <typename R, bool TIME = true>
 __device__ __forceinline__ Rhs<R> rhs(const RConst<R>& c, R r, R u) {
     R r2 = r * r;
     R Delta = r2 - R(2) * r + c.a2;
@@ -137,13 +138,15 @@ __device__ __forceinline__ Rhs<R> rhs(const RConst<R>& c, R r, R u) {
     o.d1 = R(2) * r * Pp - (r - R(1)) * c.K;
     o.d3 = u * (c.a2 * (R(1) - R(2) * u * u) - c.eta - c.lam2);
     o.d4 = -(c.a * Pp * iD + c.lam * is2 - c.a);
-    o.d5 = -((r2 + c.a2) * Pp * iD + c.a * (c.lam - c.a * s2));
+    o.d5 = TIME ? -((r2 + c.a2) * Pp * iD + c.a * (c.lam - c.a * s2)) : R(0);
     return o;
 }
 
 /* One Dormand-Prince 5(4) step with stage arithmetic in R (float: far field, double: near field),
-   the 5th-order state accumulated in FP64 and the scaled error norm in FP64. */
-template <typename R>
+   the 5th-order state accumulated in FP64 and the scaled error norm in FP64. Without TIME the
+   coordinate time t and its derivative are carried over unchanged: t is in no component of the
+   reference's error norm, so the step sequence and every other component are the same bits. */
+template <typename R, bool TIME = true>
 __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, double h, double inv_tol, double inv_tol_t,
                                        State& o) {
     using namespace bhd;
@@ -155,56 +158,60 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     R S5_0 = R(a51) * q0, S5_1 = R(a51) * q1, S5_2 = R(a51) * q2, S5_3 = R(a51) * q3;
     R S6_0 = R(a61) * q0, S6_1 = R(a61) * q1, S6_2 = R(a61) * q2, S6_3 = R(a61) * q3;
     R O0 = R(b1) * q0, O1 = R(b1) * q1, O2 = R(b1) * q2, O3 = R(b1) * q3, O4 = R(b1) * (R)s.k4,
-      O5 = R(b1) * (R)s.k5;
+      O5 = TIME ? R(b1) * (R)s.k5 : R(0);
     R E0 = R(e1) * q0, E1 = R(e1) * q1, E2 = R(e1) * q2, E3 = R(e1) * q3, E4 = R(e1) * (R)s.k4,
-      E5 = R(e1) * (R)s.k5;
+      E5 = TIME ? R(e1) * (R)s.k5 : R(0);
     const R ha = hr * R(a21);
     R t0 = y0 + ha * q0, t1 = y1 + ha * q1, t2 = y2 + ha * q2, t3 = y3 + ha * q3;
-    Rhs<R> k = rhs<R>(c, t0, t2);
+    Rhs<R> k = rhs<R, TIME>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S3_0 += R(a32) * q0; S3_1 += R(a32) * q1; S3_2 += R(a32) * q2; S3_3 += R(a32) * q3;
     S4_0 += R(a42) * q0; S4_1 += R(a42) * q1; S4_2 += R(a42) * q2; S4_3 += R(a42) * q3;
     S5_0 += R(a52) * q0; S5_1 += R(a52) * q1; S5_2 += R(a52) * q2; S5_3 += R(a52) * q3;
     S6_0 += R(a62) * q0; S6_1 += R(a62) * q1; S6_2 += R(a62) * q2; S6_3 += R(a62) * q3;
     t0 = y0 + hr * S3_0; t1 = y1 + hr * S3_1; t2 = y2 + hr * S3_2; t3 = y3 + hr * S3_3;
-    k = rhs<R>(c, t0, t2);
+    k = rhs<R, TIME>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S4_0 += R(a43) * q0; S4_1 += R(a43) * q1; S4_2 += R(a43) * q2; S4_3 += R(a43) * q3;
     S5_0 += R(a53) * q0; S5_1 += R(a53) * q1; S5_2 += R(a53) * q2; S5_3 += R(a53) * q3;
     S6_0 += R(a63) * q0; S6_1 += R(a63) * q1; S6_2 += R(a63) * q2; S6_3 += R(a63) * q3;
-    O0 += R(b3) * q0; O1 += R(b3) * q1; O2 += R(b3) * q2; O3 += R(b3) * q3; O4 += R(b3) * k.d4; O5 += R(b3) * k.d5;
-    E0 += R(e3) * q0; E1 += R(e3) * q1; E2 += R(e3) * q2; E3 += R(e3) * q3; E4 += R(e3) * k.d4; E5 += R(e3) * k.d5;
+    O0 += R(b3) * q0; O1 += R(b3) * q1; O2 += R(b3) * q2; O3 += R(b3) * q3; O4 += R(b3) * k.d4;
+    E0 += R(e3) * q0; E1 += R(e3) * q1; E2 += R(e3) * q2; E3 += R(e3) * q3; E4 += R(e3) * k.d4;
+    if (TIME) { O5 += R(b3) * k.d5; E5 += R(e3) * k.d5; }
     t0 = y0 + hr * S4_0; t1 = y1 + hr * S4_1; t2 = y2 + hr * S4_2; t3 = y3 + hr * S4_3;
-    k = rhs<R>(c, t0, t2);
+    k = rhs<R, TIME>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S5_0 += R(a54) * q0; S5_1 += R(a54) * q1; S5_2 += R(a54) * q2; S5_3 += R(a54) * q3;
     S6_0 += R(a64) * q0; S6_1 += R(a64) * q1; S6_2 += R(a64) * q2; S6_3 += R(a64) * q3;
-    O0 += R(b4) * q0; O1 += R(b4) * q1; O2 += R(b4) * q2; O3 += R(b4) * q3; O4 += R(b4) * k.d4; O5 += R(b4) * k.d5;
-    E0 += R(e4) * q0; E1 += R(e4) * q1; E2 += R(e4) * q2; E3 += R(e4) * q3; E4 += R(e4) * k.d4; E5 += R(e4) * k.d5;
+    O0 += R(b4) * q0; O1 += R(b4) * q1; O2 += R(b4) * q2; O3 += R(b4) * q3; O4 += R(b4) * k.d4;
+    E0 += R(e4) * q0; E1 += R(e4) * q1; E2 += R(e4) * q2; E3 += R(e4) * q3; E4 += R(e4) * k.d4;
+    if (TIME) { O5 += R(b4) * k.d5; E5 += R(e4) * k.d5; }
     t0 = y0 + hr * S5_0; t1 = y1 + hr * S5_1; t2 = y2 + hr * S5_2; t3 = y3 + hr * S5_3;
-    k = rhs<R>(c, t0, t2);
+    k = rhs<R, TIME>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
     S6_0 += R(a65) * q0; S6_1 += R(a65) * q1; S6_2 += R(a65) * q2; S6_3 += R(a65) * q3;
-    O0 += R(b5) * q0; O1 += R(b5) * q1; O2 += R(b5) * q2; O3 += R(b5) * q3; O4 += R(b5) * k.d4; O5 += R(b5) * k.d5;
-    E0 += R(e5) * q0; E1 += R(e5) * q1; E2 += R(e5) * q2; E3 += R(e5) * q3; E4 += R(e5) * k.d4; E5 += R(e5) * k.d5;
+    O0 += R(b5) * q0; O1 += R(b5) * q1; O2 += R(b5) * q2; O3 += R(b5) * q3; O4 += R(b5) * k.d4;
+    E0 += R(e5) * q0; E1 += R(e5) * q1; E2 += R(e5) * q2; E3 += R(e5) * q3; E4 += R(e5) * k.d4;
+    if (TIME) { O5 += R(b5) * k.d5; E5 += R(e5) * k.d5; }
     t0 = y0 + hr * S6_0; t1 = y1 + hr * S6_1; t2 = y2 + hr * S6_2; t3 = y3 + hr * S6_3;
-    k = rhs<R>(c, t0, t2);
+    k = rhs<R, TIME>(c, t0, t2);
     q0 = t1; q1 = k.d1; q2 = t3; q3 = k.d3;
-    O0 += R(b6) * q0; O1 += R(b6) * q1; O2 += R(b6) * q2; O3 += R(b6) * q3; O4 += R(b6) * k.d4; O5 += R(b6) * k.d5;
-    E0 += R(e6) * q0; E1 += R(e6) * q1; E2 += R(e6) * q2; E3 += R(e6) * q3; E4 += R(e6) * k.d4; E5 += R(e6) * k.d5;
+    O0 += R(b6) * q0; O1 += R(b6) * q1; O2 += R(b6) * q2; O3 += R(b6) * q3; O4 += R(b6) * k.d4;
+    E0 += R(e6) * q0; E1 += R(e6) * q1; E2 += R(e6) * q2; E3 += R(e6) * q3; E4 += R(e6) * k.d4;
+    if (TIME) { O5 += R(b6) * k.d5; E5 += R(e6) * k.d5; }
     o.y0 = s.y0 + (double)(hr * O0);
     o.y1 = s.y1 + (double)(hr * O1);
     o.y2 = s.y2 + (double)(hr * O2);
     o.y3 = s.y3 + (double)(hr * O3);
     o.y4 = s.y4 + (double)(hr * O4);
-    o.y5 = s.y5 + (double)(hr * O5);
-    k = rhs<R>(c, (R)o.y0, (R)o.y2);
+    o.y5 = TIME ? s.y5 + (double)(hr * O5) : s.y5;
+    k = rhs<R, TIME>(c, (R)o.y0, (R)o.y2);
     o.k1 = (double)k.d1;
     o.k3 = (double)k.d3;
     o.k4 = (double)k.d4;
-    o.k5 = (double)k.d5;
+    o.k5 = TIME ? (double)k.d5 : s.k5;
     E0 += R(e7) * (R)o.y1; E1 += R(e7) * k.d1; E2 += R(e7) * (R)o.y3; E3 += R(e7) * k.d3; E4 += R(e7) * k.d4;
-    E5 += R(e7) * k.d5;
+    if (TIME) E5 += R(e7) * k.d5;
     // scaled max-norm over r, r', u, u', phi (kerr.hpp:145-155) in FP64. Each of its scales is tol
     // times a factor a_i (a = 1 for u, 4 for phi), so the norm is max_i(|h E_i| / a_i) / tol: three
     // Newton-refined FP64 reciprocals for the state-dependent factors, and 1 / tol precomputed
@@ -219,7 +226,7 @@ __device__ __forceinline__ double dp54(const RConst<R>& c, const State& s, doubl
     m *= inv_tol;
     // coordinate time (not in the reference's norm): an absolute bound per step where the turbulence
     // depends on the emission time (inv_tol_t = 1 / bound, 0 when it does not)
-    m = fmax(m, fabs(h * (double)E5) * inv_tol_t);
+    if (TIME) m = fmax(m, fabs(h * (double)E5) * inv_tol_t);
     return m;
 }
 
@@ -515,6 +522,7 @@ struct MixedParams {
     double tol_mid;   // tolerance between r_tol and r_far
     double tol_far;   // tolerance of the far-field steps
     double inv_tol_t; // 1 / (absolute bound on a step's coordinate-time error); 0: no bound
+    int need_t;       // integrate the coordinate time t (emission read at its time, or ray records)
     double tol_relax; // emission-region tolerance of a ray after relax_after attempts
     double relax_cap; // and the factor on its transfer step cap
     int relax_after;
@@ -586,7 +594,9 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
         // relax_after attempts (an orbit near the photon sphere)
         const double inv_tol = R.s.y0 < M.r_tol ? (R.it >= M.relax_after ? M.inv_tol_relax : M.inv_tol)
                                                 : (R.s.y0 < M.r_far ? M.inv_tol_mid : M.inv_tol_far);
-        err = dp54<double>(cd, R.s, hc, inv_tol, M.inv_tol_t, n);
+        // coordinate time only where something reads it (wave-uniform: the frame's policy)
+        err = M.need_t ? dp54<double, true>(cd, R.s, hc, inv_tol, M.inv_tol_t, n)
+                       : dp54<double, false>(cd, R.s, hc, inv_tol, M.inv_tol_t, n);
         ++R.fp64_attempts;
     }
     R.k1_f32 = R.s.y0 >= M.r_f32;
