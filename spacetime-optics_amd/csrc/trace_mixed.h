@@ -298,12 +298,20 @@ __device__ __forceinline__ void synchrotron_exact(const EmisParams& E, float ln_
 // the FP32 sine and cosine (an orbiting photon's phi grows by 2 pi per turn). Split in two: the fetch
 // computes the cell and issues the loads of its corners (one slice, or the two slices around the
 // emission time); the blend interpolates them. Between the two the tracer computes what does not
-// depend on the density, so the loads' latency is hidden behind that work.
+// depend on the density, so the loads' latency is hidden behind that work. Each ray keeps the
+// corners of its last cell (slice k) in registers: consecutive samples of a ray usually fall in the
+// same cell (steps of 0.1-0.4 M against ~1 M cells near the hole), and then the fetch loads nothing,
+// so a ray running alone in its wave does not wait for memory. The values are those in memory, so
+// the result is the same bits either way.
 struct Corners {
     float c000, c100, c010, c110, c001, c101, c011, c111;
 };
+struct TurbCache {
+    Corners v;          // the corners of cell `key` (slice k)
+    unsigned int key;   // grid offset of the cell's first corner, slice included; ~0u: none
+};
 struct TurbFetch {
-    Corners k0, k1;   // slice k (and k + 1)
+    Corners k1;   // slice k + 1 (time slices only)
     float fx, fy, fz, ft;
     bool two;
 };
@@ -320,7 +328,7 @@ __device__ __forceinline__ float grid_blend_f32(const Corners& v, float fx, floa
 }
 
 __device__ __forceinline__ void turb_fetch_f32(const TraceParams& P, float R, float r, float u, double phid,
-                                               double td, TurbFetch& T) {
+                                               double td, TurbFetch& T, TurbCache& C) {
     const double two_pi = 6.283185307179586;
     const float ph = (float)(phid - two_pi * floor(phid * (1.0 / two_pi)));
     float sp, cp;
@@ -340,23 +348,25 @@ __device__ __forceinline__ void turb_fetch_f32(const TraceParams& P, float R, fl
     const unsigned int cell = (unsigned int)iz * sz + (unsigned int)iy * sy + (unsigned int)ix;
     T.two = P.turb_nt > 1;
     // the slice (pair) and its weight first; the corner loads then write the same fields on both
-    // paths, so the struct stays in registers (no store through a selected address)
-    const float* G = P.turb + cell;
-    unsigned int slice = 0;
+    // paths, so the structs stay in registers (no store through a selected address)
+    unsigned int key = cell, slice = 0;
     T.ft = 0.f;
     if (T.two) {
         const float st = fminf(fmaxf((float)((td - P.turb_t0) * P.turb_inv_dt), 0.f), (float)(P.turb_nt - 1));
         const int k = min((int)st, P.turb_nt - 2);
         T.ft = st - (float)k;
         slice = (unsigned int)(P.turb_n * P.turb_n * P.turb_n);
-        G += (unsigned int)k * slice;
+        key += (unsigned int)k * slice;
     }
-    T.k0 = grid_corners_f32(G, sy, sz);
-    if (T.two) T.k1 = grid_corners_f32(G + slice, sy, sz);   // read only when two (a copy would wait for k0)
+    if (key != C.key) {   // a new cell: the loads refill the ray's cached corners in place
+        C.v = grid_corners_f32(P.turb + key, sy, sz);
+        C.key = key;
+    }
+    if (T.two) T.k1 = grid_corners_f32(P.turb + key + slice, sy, sz);   // read only when two
 }
 
-__device__ __forceinline__ float turb_blend_f32(const TurbFetch& T) {
-    const float A = grid_blend_f32(T.k0, T.fx, T.fy, T.fz);
+__device__ __forceinline__ float turb_blend_f32(const TurbFetch& T, const TurbCache& C) {
+    const float A = grid_blend_f32(C.v, T.fx, T.fy, T.fz);
     if (!T.two) return A;
     const float B = grid_blend_f32(T.k1, T.fx, T.fy, T.fz);
     return A + (B - A) * T.ft;
@@ -398,7 +408,7 @@ struct FlowPending {
 template <int RADIO>
 __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParams& E, double rd, double ud,
                                            double phid, double td, double lamd, double Ecamd, double hd,
-                                           FlowSample& f, FlowPending& q) {
+                                           FlowSample& f, FlowPending& q, TurbCache& C) {
     const float r = (float)rd, u = (float)ud, lam = (float)lamd;
     // sin theta and its reciprocal from one reciprocal square root
     const float s2 = fmaxf(1.f - u * u, 1e-12f), is = __builtin_amdgcn_rsqf(s2), s = s2 * is;
@@ -408,7 +418,7 @@ __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParam
     const float lnvert = -0.5f * zh * zh;
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
     if constexpr (RADIO == bhd::RADIO_GRID)
-        if (P.turb_on) turb_fetch_f32(P, R, r, u, phid, td, q.tf);   // loads in flight until flow_finish
+        if (P.turb_on) turb_fetch_f32(P, R, r, u, phid, td, q.tf, C);   // loads in flight until flow_finish
     const float lr = lnf(r);
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
@@ -453,11 +463,11 @@ __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParam
 
 template <int RADIO>
 __device__ __forceinline__ bool flow_finish(const TraceParams& P, const EmisParams& E, double rd, double phid,
-                                            double td, FlowSample& f, const FlowPending& q) {
+                                            double td, FlowSample& f, const FlowPending& q, const TurbCache& C) {
     float dens = 1.f;   // turbulence factor x horizon softening: one logarithm for both
     if (P.turb_on) {
         if constexpr (RADIO == bhd::RADIO_SHEARED) dens = sheared_factor_f32(P, (float)rd, q.R, q.z, phid, td);
-        else dens = turb_blend_f32(q.tf);
+        else dens = turb_blend_f32(q.tf, C);
     }
     const float ln_ne = E.ln_n0 - E.n_index * q.lr + q.lnvert + lnf(dens * q.soft);
     if (ln_ne < -69.07755f) return false;   // n_e < 1e-30
@@ -540,6 +550,7 @@ struct RayStateM {
     State s;
     double h, acc0, acc1, acc2, acc3, trans;
     float psi, s_sin;   // swept angle; sin theta of the current state
+    TurbCache tc;       // the turbulence corners of the ray's last flow sample
     int ncross, accepted, flow_steps, it, fate, fp64_attempts;
     bool done, k1_f32;
 };
@@ -564,6 +575,8 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
     R.psi = 0.f;
     R.s_sin = sqrtf_(fmaxf((float)(1.0 - R.s.y2 * R.s.y2), 0.f));
     R.ncross = R.accepted = R.flow_steps = R.it = R.fate = R.fp64_attempts = 0;
+    R.tc.v = Corners{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    R.tc.key = ~0u;   // grids hold fewer than 2^31 cells: no cell has this offset
     R.done = P.max_steps <= 0;
 }
 
@@ -614,7 +627,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     FlowSample f;
     FlowPending q;
     bool emits = false;
-    if (in_flow) emits = flow_begin<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f, q);
+    if (in_flow) emits = flow_begin<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f, q, R.tc);
     // sin theta of the new state (FP64 1 - u^2, then FP32): the ring angle and the next step cap
     const float s1 = sqrtf_(fmaxf((float)(1.0 - n.y2 * n.y2), 0.f));
     if (RADIO) {
@@ -631,7 +644,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
     if (in_flow) {
         ++R.flow_steps;
-        if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q)) {
+        if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q, R.tc)) {
             float ln_j, ln_a, ln_bnu, fj, fa;
             if (M.use_surrogate &&
                 surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
