@@ -489,39 +489,50 @@ __device__ __forceinline__ void transfer(const EmisParams& E, const FlowSample& 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Surrogate lookup: (ln f_j, ln f_a) of the tabulated network at (u0 = log10 nu/nu_c,
-// log10 Theta, theta_B folded into [0, pi/2]); false outside the box or in a cell the exact routine
-// judged inaccurate (the caller then evaluates the exact formula).
-// Trilinear lookup of the baked surrogate at surrogate coordinates (ln X, ln Theta, ln sin theta_B);
-// false outside the table or in a cell the exact-routine judge flagged. ln X above the table is
-// clamped to its edge, where both coefficients sit at their floors.
-__device__ __forceinline__ bool surrogate_lookup(const bhmlp::SurrogateTable& T, float lx, float lth, float ls,
-                                                 float* fj, float* fa) {
+// Trilinear lookup of the baked surrogate at surrogate coordinates (ln X, ln Theta, ln sin theta_B),
+// in two parts so the caller can evaluate the exact formula while the table's loads are in flight:
+// the fetch computes the cell and issues the loads of its trust flag and eight corners (one memory
+// round trip; the indices are in range whatever the flag says); the blend interpolates them. Outside
+// the table, or in a cell the exact-routine judge flagged, the exact value stands. ln X above the
+// table is clamped to its edge, where both coefficients sit at their floors.
+struct SurrogateFetch {
+    float2 v000, v100, v010, v110, v001, v101, v011, v111;
+    float f0, f1, f2;
+    bool use;   // inside the table and its cell not flagged (valid after the loads)
+};
+
+__device__ __forceinline__ void surrogate_fetch(const bhmlp::SurrogateTable& T, float lx, float lth, float ls,
+                                                SurrogateFetch& F) {
     const float g0 = (fminf(lx, T.hi0) - T.lo0) * T.inv_d0, g1 = (lth - T.lo1) * T.inv_d1, g2 = (ls - T.lo2) * T.inv_d2;
-    if (!(g0 >= 0.f && g1 >= 0.f && g2 >= 0.f && g0 <= (float)(T.n0 - 1) && g1 <= (float)(T.n1 - 1) &&
-          g2 <= (float)(T.n2 - 1)))
-        return false;
-    const int i0 = min((int)g0, T.n0 - 2), i1 = min((int)g1, T.n1 - 2), i2 = min((int)g2, T.n2 - 2);
-    // the cell's trust flag and its eight corners are loaded together (the indices are in range
-    // whatever the flag says), so the lookup costs one memory round trip; 32-bit offsets (bh_mlp_bake
-    // keeps the table at most 2^24 nodes)
+    const bool in = g0 >= 0.f && g1 >= 0.f && g2 >= 0.f && g0 <= (float)(T.n0 - 1) && g1 <= (float)(T.n1 - 1) &&
+                    g2 <= (float)(T.n2 - 1);
+    // clamped in float before the conversion, so any input (NaN included) gives a cell in range;
+    // inside the box this is the cell of the point (outside, its values are not used)
+    const int i0 = (int)fminf(fmaxf(g0, 0.f), (float)(T.n0 - 2)), i1 = (int)fminf(fmaxf(g1, 0.f), (float)(T.n1 - 2)),
+              i2 = (int)fminf(fmaxf(g2, 0.f), (float)(T.n2 - 2));
+    // 32-bit offsets: bh_mlp_bake keeps the table at most 2^24 nodes
     const unsigned int s1 = (unsigned int)T.n0, s2 = (unsigned int)(T.n0 * T.n1);
     const unsigned char flag = T.bad[((unsigned int)i2 * (unsigned int)(T.n1 - 1) + (unsigned int)i1) *
                                          (unsigned int)(T.n0 - 1) + (unsigned int)i0];
     const float2* b = (const float2*)T.tab + ((unsigned int)i2 * s2 + (unsigned int)i1 * s1 + (unsigned int)i0);
-    const float2 v000 = b[0], v100 = b[1], v010 = b[s1], v110 = b[s1 + 1];
-    const float2 v001 = b[s2], v101 = b[s2 + 1], v011 = b[s2 + s1], v111 = b[s2 + s1 + 1];
-    if (flag) return false;
-    const float f0 = g0 - i0, f1 = g1 - i1, f2 = g2 - i2;
-    float x00 = v000.x + (v100.x - v000.x) * f0, x10 = v010.x + (v110.x - v010.x) * f0;
-    float x01 = v001.x + (v101.x - v001.x) * f0, x11 = v011.x + (v111.x - v011.x) * f0;
-    float y00 = v000.y + (v100.y - v000.y) * f0, y10 = v010.y + (v110.y - v010.y) * f0;
-    float y01 = v001.y + (v101.y - v001.y) * f0, y11 = v011.y + (v111.y - v011.y) * f0;
+    F.v000 = b[0]; F.v100 = b[1]; F.v010 = b[s1]; F.v110 = b[s1 + 1];
+    F.v001 = b[s2]; F.v101 = b[s2 + 1]; F.v011 = b[s2 + s1]; F.v111 = b[s2 + s1 + 1];
+    F.f0 = g0 - (float)i0;
+    F.f1 = g1 - (float)i1;
+    F.f2 = g2 - (float)i2;
+    F.use = in && !flag;
+}
+
+__device__ __forceinline__ void surrogate_blend(const SurrogateFetch& F, float* fj, float* fa) {
+    const float f0 = F.f0, f1 = F.f1, f2 = F.f2;
+    float x00 = F.v000.x + (F.v100.x - F.v000.x) * f0, x10 = F.v010.x + (F.v110.x - F.v010.x) * f0;
+    float x01 = F.v001.x + (F.v101.x - F.v001.x) * f0, x11 = F.v011.x + (F.v111.x - F.v011.x) * f0;
+    float y00 = F.v000.y + (F.v100.y - F.v000.y) * f0, y10 = F.v010.y + (F.v110.y - F.v010.y) * f0;
+    float y01 = F.v001.y + (F.v101.y - F.v001.y) * f0, y11 = F.v011.y + (F.v111.y - F.v011.y) * f0;
     float x0 = x00 + (x10 - x00) * f1, x1 = x01 + (x11 - x01) * f1;
     float y0 = y00 + (y10 - y00) * f1, y1 = y01 + (y11 - y01) * f1;
     *fj = x0 + (x1 - x0) * f2;
     *fa = y0 + (y1 - y0) * f2;
-    return true;
 }
 
 struct MixedParams {
@@ -645,13 +656,21 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     if (in_flow) {
         ++R.flow_steps;
         if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q, R.tc)) {
-            float ln_j, ln_a, ln_bnu, fj, fa;
-            if (M.use_surrogate &&
-                surrogate_lookup(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
-                                 f.ln_sinb, &fj, &fa)) {
-                ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
-                ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
-                ln_bnu = ln_j - ln_a;   // both within the table's floors: no cancellation
+            float ln_j, ln_a, ln_bnu;
+            if (M.use_surrogate) {
+                // the table's loads first; the exact formula runs while they are in flight, and the
+                // table replaces its values wherever the cell is trusted
+                SurrogateFetch sf;
+                surrogate_fetch(M.S, f.ln_nu - f.ln_nuc - (float)bhmlp::LN_2_9 - 2.f * f.ln_th - f.ln_sinb, f.ln_th,
+                                f.ln_sinb, sf);
+                synchrotron_exact(M.E, f.ln_ne, f.ln_th, f.ln_k2, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a, &ln_bnu);
+                if (sf.use) {
+                    float fj, fa;
+                    surrogate_blend(sf, &fj, &fa);
+                    ln_j = f.ln_ne + M.ln_e2_over_c + f.ln_nuc + fj;
+                    ln_a = f.ln_ne + M.ln_e2_over_mec - f.ln_nuc + fa;
+                    ln_bnu = ln_j - ln_a;   // both within the table's floors: no cancellation
+                }
             } else {
                 synchrotron_exact(M.E, f.ln_ne, f.ln_th, f.ln_k2, f.ln_nu, f.ln_nuc, f.ln_sinb, &ln_j, &ln_a, &ln_bnu);
             }
