@@ -52,10 +52,12 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
+// Two waves per SIMD at least: with 256-lane workgroups alone the compiler may take VGPRs and AGPRs
+// beyond 256 together (one wave per SIMD); the bound makes it spill to scratch instead.
 template <int PREC, int RADIO, bool RAYS>
-__global__ __launch_bounds__(BLOCK) void k_trace(const TraceParams P, const bhm32::MixedParams M,
-                                                 float4* __restrict__ px_out, bh_ray_out* __restrict__ rays_out,
-                                                 BlockOut* __restrict__ blk, int per_xcd) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_trace(
+    const TraceParams P, const bhm32::MixedParams M, float4* __restrict__ px_out, bh_ray_out* __restrict__ rays_out,
+    BlockOut* __restrict__ blk, int per_xcd) {
     const int tiles_x = (P.w + TILE - 1) / TILE, tiles_y = (P.h + TILE - 1) / TILE;
     const int tile = xcd_tile(blockIdx.x, tiles_x * tiles_y, per_xcd);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

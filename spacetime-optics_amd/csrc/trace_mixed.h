@@ -327,6 +327,7 @@ __device__ __forceinline__ float grid_blend_f32(const Corners& v, float fx, floa
     return c0 + (c1 - c0) * fz;
 }
 
+template <bool CACHE>
 __device__ __forceinline__ void turb_fetch_f32(const TraceParams& P, float R, float r, float u, double phid,
                                                double td, TurbFetch& T, TurbCache& C) {
     const double two_pi = 6.283185307179586;
@@ -358,7 +359,7 @@ __device__ __forceinline__ void turb_fetch_f32(const TraceParams& P, float R, fl
         slice = (unsigned int)(P.turb_n * P.turb_n * P.turb_n);
         key += (unsigned int)k * slice;
     }
-    if (key != C.key) {   // a new cell: the loads refill the ray's cached corners in place
+    if (!CACHE || key != C.key) {   // a new cell: the loads refill the ray's cached corners in place
         C.v = grid_corners_f32(P.turb + key, sy, sz);
         C.key = key;
     }
@@ -405,7 +406,7 @@ struct FlowPending {
     TurbFetch tf;
 };
 
-template <int RADIO>
+template <int RADIO, bool TCACHE>
 __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParams& E, double rd, double ud,
                                            double phid, double td, double lamd, double Ecamd, double hd,
                                            FlowSample& f, FlowPending& q, TurbCache& C) {
@@ -418,7 +419,7 @@ __device__ __forceinline__ bool flow_begin(const TraceParams& P, const EmisParam
     const float lnvert = -0.5f * zh * zh;
     if (lnvert < -16.118096f) return false;   // vert < 1e-7
     if constexpr (RADIO == bhd::RADIO_GRID)
-        if (P.turb_on) turb_fetch_f32(P, R, r, u, phid, td, q.tf, C);   // loads in flight until flow_finish
+        if (P.turb_on) turb_fetch_f32<TCACHE>(P, R, r, u, phid, td, q.tf, C);   // loads in flight until flow_finish
     const float lr = lnf(r);
     const float dr = r - E.rh;
     const float soft = 1.f - expf_(-dr * dr * 4.f);
@@ -591,8 +592,10 @@ __device__ __forceinline__ void ray_init_m(const TraceParams& P, const MixedPara
     R.done = P.max_steps <= 0;
 }
 
-// One attempt of trace.cuh:81-163 under the mixed policy.
-template <int RADIO>
+// One attempt of trace.cuh:81-163 under the mixed policy. TCACHE keeps the ray's turbulence cell
+// in its state across attempts (the persistent engine); without it every flow sample loads its
+// corners (the frame kernel, already at the register limit, would spill the cache in its loop).
+template <int RADIO, bool TCACHE = true>
 __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedParams& M, RayStateM& R) {
     const bool flow = RADIO && P.flow_on;
     State n;
@@ -637,8 +640,10 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     const bool in_flow = flow && n.y0 < M.r_cut;
     FlowSample f;
     FlowPending q;
+    TurbCache tl;
+    TurbCache& tc = TCACHE ? R.tc : tl;
     bool emits = false;
-    if (in_flow) emits = flow_begin<RADIO>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f, q, R.tc);
+    if (in_flow) emits = flow_begin<RADIO, TCACHE>(P, M.E, n.y0, n.y2, n.y4, n.y5, R.c.lam, R.c.E, hc, f, q, tc);
     // sin theta of the new state (FP64 1 - u^2, then FP32): the ring angle and the next step cap
     const float s1 = sqrtf_(fmaxf((float)(1.0 - n.y2 * n.y2), 0.f));
     if (RADIO) {
@@ -655,7 +660,7 @@ __device__ __forceinline__ void ray_step_m(const TraceParams& P, const MixedPara
     if ((R.s.y2 > 0.0) != (n.y2 > 0.0)) ++R.ncross;
     if (in_flow) {
         ++R.flow_steps;
-        if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q, R.tc)) {
+        if (emits && flow_finish<RADIO>(P, M.E, n.y0, n.y4, n.y5, f, q, tc)) {
             float ln_j, ln_a, ln_bnu;
             if (M.use_surrogate) {
                 // the table's loads first; the exact formula runs while they are in flight, and the
@@ -749,7 +754,7 @@ __device__ __forceinline__ int trace_ray_mixed(const TraceParams& P, const Mixed
                                                RayResult& out) {
     RayStateM R;
     ray_init_m<RADIO>(P, M, x, yv, R);
-    while (!R.done) ray_step_m<RADIO>(P, M, R);
+    while (!R.done) ray_step_m<RADIO, false>(P, M, R);
     return ray_result_m(P, R, out);
 }
 
