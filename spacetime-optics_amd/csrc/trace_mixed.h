@@ -7,7 +7,9 @@
      r_near, near the hole) - there the reference's steps are tolerance-limited and the
      radiative-transfer quadrature depends on that step sequence; tol_mid from r_tol to r_far (the
      flow's edge in radio mode); tol_far beyond (radio: scaled with the pixel's size, 1e-4 on
-     0.04 M pixels; vacuum: 1e-6). Outside r_tol the step-size controller also scales the next step
+     0.04 M pixels, and by sin(theta_cam) / sin(15 deg) for a camera within 15 degrees of the spin
+     axis, where the norm's bound on cos(theta) is a looser bound on theta; vacuum: 1e-6). Outside
+     r_tol the step-size controller also scales the next step
      by r_start / r_end of the accepted one (the 1/r scaling of the Mino-time step far from the
      hole): the reference's error-only controller lags behind it and rejected every other step on
      the way out (CPU sweep: 114 -> 95 attempts per ray, rejections 18 -> 2.6, errors unchanged).
@@ -18,14 +20,17 @@
    - Stage precision: FP64 stages evaluated with fused multiply-adds and Newton-refined FP64
      reciprocals; optionally (r >= r_f32) DP5(4) stages in FP32 with hardware reciprocals, the
      state accumulated in FP64, the error norm, step controller and radial-constraint projection
-     in FP64. The projection r' = +-sqrt(R(r)) after every accepted step keeps the radial motion on
-     the photon's constant-of-motion surface, so FP32 stage errors only shift where along the
-     trajectory a step lands. The FP32 stages are off by default: without packed FP32 the
-     arithmetic runs at the FP64 rate, and lanes of a wave crossing r_f32 at different steps run
-     both stage codes.
+     in FP64. The coordinate time is integrated only where something reads it (time-dependent
+     turbulence, a t bound, ray records). The projection r' = +-sqrt(R(r)) after every accepted
+     step keeps the radial motion on the photon's constant-of-motion surface, so FP32 stage errors
+     only shift where along the trajectory a step lands. The FP32 stages are off by default:
+     without packed FP32 the arithmetic runs at the FP64 rate, and lanes of a wave crossing r_f32
+     at different steps run both stage codes.
    - Emission and absorption in FP32, in log space (the cgs coefficients span 1e-45..1e-10), exact
      thermal-synchrotron formula or the MLP surrogate table; intensity and transmission accumulate
-     in FP64. The swept angle (photon-ring order) is tracked in FP32.
+     in FP64. The swept angle (photon-ring order) is tracked in FP32. In the persistent engine each
+     ray keeps its last turbulence cell in registers, so consecutive samples in one cell load
+     nothing; the surrogate's table loads are in flight while the exact formula runs.
    None of this is bit-compatible with the FP64 tracer; its error against the FP64 oracle is what
    the tests bound. */
 #pragma once
