@@ -122,13 +122,13 @@ typedef struct bh_trace_opts {
                          vacuum with tol_mid = max(max(1e-6, tol), tol_far / 10); 1 -> emission,
                          absorption and the step cap over the whole flow as the FP64 tracer, with
                          tol_mid = max(1e-6, tol) */
-    int engine;       /* bh_render / bh_render_resident: BH_ENGINE_AUTO (0) -> the persistent engine
-                         (bh_autotune's last result for this precision, or the defaults): one sample per
-                         pixel as bh_trace_batch of one frame, more as the first pass of
-                         bh_render_adaptive alone; frames beyond 2^31 samples, or whose samples
-                         would take more than 1 GiB (32 bytes each), on the frame kernel.
-                         BH_ENGINE_FRAME -> always the frame kernel. Both give the same pixels and
-                         counters. */
+    int engine;       /* bh_render / bh_render_resident / bh_trace_tile: BH_ENGINE_AUTO (0) -> the
+                         persistent engine (bh_autotune's last result for this precision, or the
+                         defaults): one sample per pixel as bh_trace_batch of one frame, more as the
+                         first pass of bh_render_adaptive alone; frames beyond 2^31 samples, or whose
+                         samples would take more than 1 GiB (32 bytes each), on the frame kernel.
+                         BH_ENGINE_FRAME -> always the frame kernel. Both give the same pixels,
+                         counters and ray records. */
     double tol_t;     /* mixed: absolute bound (M) on one step's coordinate-time error, which the
                          reference's error norm leaves out; it matters where the turbulence depends on
                          the emission time (grid time slices, the sheared model). 0 -> the default:
@@ -205,7 +205,9 @@ int bh_render_noalloc(bh_ctx* ctx, const bh_frame* frame, const bh_trace_opts* o
 
 /* Tile interface (k_pass1's per-pixel launch, render.cu:89-128, with every per-ray output): rays of
    pixels x0 <= px < x0+w, y0 <= py < y0+h, one sample each (centre or jittered sample 0), written
-   row-major to out[w*h]. */
+   row-major to out[w*h]. opts->engine BH_ENGINE_AUTO (0): the persistent engine with bh_autotune's
+   last launch for the precision (or the defaults); BH_ENGINE_FRAME: the frame kernel. Both give the
+   same records. */
 int bh_trace_tile(bh_ctx* ctx, const bh_frame* frame, int x0, int y0, int w, int h, const bh_trace_opts* opts,
                   bh_ray_out* out);
 

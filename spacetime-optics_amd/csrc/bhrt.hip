@@ -93,21 +93,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
             flux = a0;
             const size_t o = (size_t)ty * P.w + tx;
             if (RAYS) {
-                bh_ray_out* q = rays_out + o;
-                for (int i = 0; i < 6; ++i) q->y[i] = r.y[i];
-                q->th_inf = r.th_inf;
-                q->ph_inf = r.ph_inf;
-                q->I[0] = r.I[0]; q->I[1] = r.I[1]; q->I[2] = r.I[2]; q->I[3] = r.I[3];
-                q->trans = r.trans;
-                q->psi = r.psi;
-                q->fate = r.fate;
-                q->ncross = r.ncross;
-                q->order = r.order;
-                q->steps = r.steps;
-                q->accepted = r.accepted;
-                q->flow_steps = r.flow_steps;
-                q->sky_u = r.sky_u;
-                q->sky_v = r.sky_v;
+                bhp::write_record(rays_out + o, r);
             } else {
                 const double m = (double)P.spp;   // mean as sum / count, as the adaptive resolve and the oracle
                 px_out[o] = make_float4((float)(a0 / m), (float)(a1 / m), (float)(a2 / m), (float)(a3 / m));
@@ -420,7 +406,7 @@ __global__ __launch_bounds__(256) void k_order_tiles(const bhp::BatchFrame* __re
     px = min(px, F.P.w - 1);
     py = min(py, F.P.h - 1);
     double ox, oy;
-    bhd::sample_offsets(F.P, px, py, 0, &ox, &oy);
+    bhd::sample_offsets(F.P, F.P.x0 + px, F.P.y0 + py, 0, &ox, &oy);   // x0 = y0 = 0 but for tile records
     bhd::RayConst rc;
     bhd::State st;
     bhd::camera_photon(F.P, ox, oy, rc, st);
@@ -1209,6 +1195,64 @@ int launch_batch(bh_ctx* c, const bh_batch_frame* frames, int n, const bh_trace_
     return BH_OK;
 }
 
+// Ray records of a w x h region at (x0, y0) (bh_trace_tile) with one persistent launch: the region's
+// 8 x 8 tiles are the queue entries, in the launch configuration's order; the records land in d_rays.
+// The rays and their records are those of the frame kernel (same per-ray code).
+int launch_records(bh_ctx* c, const bh_frame* f, const bh_trace_opts* opts, int x0, int y0, int w, int h) {
+    int rc = check_frame(c, f);
+    if (rc) return rc;
+    if (w <= 0 || h <= 0) return fail(c, BH_ERR_ARG, "empty tile");
+    if ((long long)w * h > BATCH_MAX_PIXELS) return fail(c, BH_ERR_ARG, "tile too large (2^30 rays)");
+    bh_trace_opts o{};
+    if (opts) o = *opts;
+    rc = check_opts(c, o);
+    if (rc) return rc;
+    bh_batch_cfg cfg = c->tuned_ok[o.precision] ? c->tuned[o.precision] : batch_defaults(o.precision);
+    rc = resolve_cfg(c, o.precision, &cfg);
+    if (rc) return rc;
+    rc = prepare_turbulence(c, c->S, f);
+    if (rc) return rc;
+    const auto k = pick_adaptive<bhp::RecordWork>(o.precision, radio_kind(c, c->S.mode), cfg.waves);
+    const int grid = persist_grid(c, (const void*)k, cfg.block, cfg.grid_per_cu);
+    const int tiles_x = (w + 7) / 8, tiles = tiles_x * ((h + 7) / 8);
+    bhp::BatchFrame B;
+    std::memset(&B, 0, sizeof B);
+    B.P = make_params(c, f);
+    B.P.x0 = x0;
+    B.P.y0 = y0;
+    B.P.w = w;
+    B.P.h = h;
+    B.P.spp = 1;
+    B.P.jitter = o.jitter;
+    B.M = make_mixed(c, c->S, f, o);
+    B.M.need_t = 1;   // the records carry t
+    if (o.emissivity == BH_EMIS_MLP) {
+        B.M.use_surrogate = 1;
+        B.M.S = c->stab;
+    }
+    B.tiles_x = tiles_x;
+    B.nentries = (unsigned int)tiles * 64u;
+    const bool ordered = cfg.order != BH_ORDER_ROWS;
+    if (ordered) {
+        if ((rc = ensure(c, &c->d_order, &c->order_cap, (size_t)tiles)) ||
+            (rc = ensure(c, &c->d_ohist, &c->ohist_cap, (size_t)ORDER_BUCKETS)))
+            return rc;
+        B.order = c->d_order;
+    }
+    if ((rc = ensure(c, &c->d_bframes, &c->bframes_cap, sizeof B)) || (rc = ensure(c, &c->d_queue, &c->queue_cap, 1)) ||
+        (rc = ensure(c, &c->d_blk, &c->blk_cap, (size_t)grid)) ||
+        (rc = ensure(c, &c->d_rays, &c->rays_cap, (size_t)w * h)))
+        return rc;
+    const bhp::BatchFrame* dB = (const bhp::BatchFrame*)c->d_bframes;
+    HIPCK(c, hipMemcpyAsync(c->d_bframes, &B, sizeof B, hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->st, c->d_queue, 1);
+    if (ordered) enqueue_tile_order(c->st, dB, 1, (unsigned int)tiles, c->d_ohist, cfg.order);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(cfg.block), 0, c->st, bhp::RecordWork{dB, 1, c->d_rays}, c->d_queue,
+                       cfg.interval, cfg.refill_min, cfg.prio_age > 0 ? cfg.prio_age : 0, c->d_blk);
+    HIPCK(c, hipGetLastError());
+    return BH_OK;
+}
+
 // Adaptive render (render.cu:677-721): persistent pass 1 over (tile, sample, pixel) entries, the
 // statistics and decisions on the device, one host read of the pass-2 size, persistent pass 2 over
 // chunks, resolve. px_out / nsamp_out are device pointers (d_px, d_nsamp).
@@ -1599,7 +1643,9 @@ int bh_trace_tile(bh_ctx* c, const bh_frame* f, int x0, int y0, int w, int h, co
     if (!c) return BH_ERR_ARG;
     if (!out) return fail(c, BH_ERR_ARG, "output is NULL");
     HIPCK(c, hipSetDevice(c->device));
-    int rc = launch_trace(c, f, opts, x0, y0, w, h, true, nullptr);
+    const bool frame_kernel = opts && opts->engine == BH_ENGINE_FRAME;
+    int rc = frame_kernel ? launch_trace(c, f, opts, x0, y0, w, h, true, nullptr)
+                          : launch_records(c, f, opts, x0, y0, w, h);
     if (rc) return rc;
     HIPCK(c, hipMemcpyAsync(out, c->d_rays, sizeof(bh_ray_out) * (size_t)w * h, hipMemcpyDeviceToHost, c->st));
     HIPCK(c, hipStreamSynchronize(c->st));

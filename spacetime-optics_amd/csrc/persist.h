@@ -13,8 +13,8 @@
    max_steps steps, so the grid always drains. With prio_age > 0 a wave that holds a ray older than
    prio_age steps raises its issue priority, so the longest rays run at their own latency instead
    of sharing the SIMD evenly with younger waves. What an entry means (which pixel and sample, where
-   the finished ray is written) is a Work policy: frame batches, and the two passes of adaptive
-   sampling. */
+   the finished ray is written) is a Work policy: frame batches, the tile interface's ray records,
+   and the two passes of adaptive sampling. */
 #pragma once
 #include "trace_fp64.h"
 #include "trace_mixed.h"
@@ -62,6 +62,47 @@ struct BatchWork {
         tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e >> 6), e & 63u, &px, &py);
         frames[f].out[(size_t)py * frames[f].P.w + px] =
             make_float4((float)r.I[0], (float)r.I[1], (float)r.I[2], (float)r.I[3]);
+    }
+};
+
+// Every per-ray output of a finished ray as the tile interface's record.
+__device__ __forceinline__ void write_record(bh_ray_out* __restrict__ q, const RayResult& r) {
+    for (int i = 0; i < 6; ++i) q->y[i] = r.y[i];
+    q->th_inf = r.th_inf;
+    q->ph_inf = r.ph_inf;
+    q->I[0] = r.I[0]; q->I[1] = r.I[1]; q->I[2] = r.I[2]; q->I[3] = r.I[3];
+    q->trans = r.trans;
+    q->psi = r.psi;
+    q->fate = r.fate;
+    q->ncross = r.ncross;
+    q->order = r.order;
+    q->steps = r.steps;
+    q->accepted = r.accepted;
+    q->flow_steps = r.flow_steps;
+    q->sky_u = r.sky_u;
+    q->sky_v = r.sky_v;
+}
+
+// Ray records of a w x h region at (x0, y0) of a frame (the tile interface): P.w / P.h are the
+// region's size and P.x0 / P.y0 its origin; one ray per pixel (centre, or jittered sample 0), its
+// record at out[(py - y0) * w + (px - x0)].
+struct RecordWork {
+    const BatchFrame* frames;   // one frame
+    int n;
+    bh_ray_out* out;
+    __device__ __forceinline__ bool begin(int f, unsigned int e, int* px, int* py, int* s) const {
+        const TraceParams& P = frames[f].P;
+        int lx, ly;
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e >> 6), e & 63u, &lx, &ly);
+        *px = P.x0 + lx;
+        *py = P.y0 + ly;
+        *s = 0;
+        return lx < P.w && ly < P.h;
+    }
+    __device__ __forceinline__ void finish(int f, unsigned int e, const RayResult& r) const {
+        int lx, ly;
+        tile_pixel(frames[f].tiles_x, queued_tile(frames[f], e >> 6), e & 63u, &lx, &ly);
+        write_record(out + (size_t)ly * frames[f].P.w + lx, r);
     }
 };
 

@@ -224,3 +224,49 @@ def test_render_many_samples_beyond_the_sample_budget():
     assert_same([got], [want], "spp 9 at 2048^2")
     assert np.array_equal(stats_vec(st), stats_vec(sf)) and st.samples == 9 * 2048 * 2048
     t.close()
+
+
+def assert_records_same(a, b, what):
+    assert a.shape == b.shape, what
+    for f in a.dtype.names:
+        x, y = np.ascontiguousarray(a[f]), np.ascontiguousarray(b[f])
+        neq = (x.view(np.uint8) != y.view(np.uint8)).reshape(x.shape[0], x.shape[1], -1).any(axis=-1)
+        bad = np.argwhere(neq)
+        assert len(bad) == 0, f"{what}: field {f} differs at {len(bad)} rays, first {bad[:3].tolist()}"
+
+
+@pytest.mark.parametrize("prec", [bh.PREC_FP64, bh.PREC_MIXED])
+def test_trace_tile_engines_agree(c2, prec):
+    """bh_trace_tile runs on the persistent engine by default (the region's 8 x 8 tiles as queue
+    entries, in the launch's order: rows for FP64, critical for mixed); the frame kernel (engine
+    BH_ENGINE_FRAME) must give the same records bit for bit: the whole frame and a ragged region
+    away from the origin, centre and jittered rays, the radio frame, the vacuum frame and
+    time-sliced turbulence (where the mixed tracer bounds the coordinate-time error)."""
+    cfg, t = c2
+    regions = ((0, 0, 256, 256), (77, 101, 37, 29))
+    for jitter in (0, 1):
+        for x0, y0, w, h in regions:
+            got = t.trace_tile(cfg["F"], x0, y0, w, h, t.opts(precision=prec, jitter=jitter))
+            want = t.trace_tile(cfg["F"], x0, y0, w, h, t.opts(precision=prec, jitter=jitter, engine=bh.ENGINE_FRAME))
+            assert_records_same(got, want, f"radio prec {prec} jitter {jitter} region {(x0, y0, w, h)}")
+            assert (got["flow_steps"] > 0).any()
+    vac = setup(shot("c1_vacuum_256.ini"))
+    tv = tracer_for(vac)
+    try:
+        for x0, y0, w, h in regions:
+            got = tv.trace_tile(vac["F"], x0, y0, w, h, tv.opts(precision=prec))
+            want = tv.trace_tile(vac["F"], x0, y0, w, h, tv.opts(precision=prec, engine=bh.ENGINE_FRAME))
+            assert_records_same(got, want, f"vacuum prec {prec} region {(x0, y0, w, h)}")
+    finally:
+        tv.close()
+    ts = setup(shot("c2_radio_1024.ini"), W=96, H=96)
+    grid = bh.turbulence(64, 4, 60.0, 0.3, 5.0, 30.0, 10.0, 3)
+    tt = bh.Tracer(ts["S"], k2=ts["k2"], k2_l0=ts["l0"], k2_dl=ts["dl"], grid=grid, turb_half=60.0,
+                   turb_t0=-1060.0, turb_dt=10.0)
+    try:
+        F = frame_for(ts["S"], shot("c2_radio_1024.ini"), 96, 96, t=15.0)
+        got = tt.trace_tile(F, 5, 9, 83, 80, tt.opts(precision=prec))
+        want = tt.trace_tile(F, 5, 9, 83, 80, tt.opts(precision=prec, engine=bh.ENGINE_FRAME))
+        assert_records_same(got, want, f"time slices prec {prec}")
+    finally:
+        tt.close()
