@@ -16,6 +16,7 @@
    the finished ray is written) is a Work policy: frame batches, the tile interface's ray records,
    and the two passes of adaptive sampling. */
 #pragma once
+#include "../../include/bhrt.h"
 #include "trace_fp64.h"
 #include "trace_mixed.h"
 
