@@ -71,7 +71,7 @@ def test_random_scene_tile_bit_exact(kind, seed, oracle_bhm):
         assert (gpu["flow_steps"] > 0).any()
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(30))
 def test_random_scene_mixed_within_bars(seed):
     """The mixed tracer's policy (emission radius from each scene's own flow model, far-field
     tolerance from its pixel size and the camera's angle to the spin axis) on the random radio
