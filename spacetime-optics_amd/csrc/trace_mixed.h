@@ -303,11 +303,11 @@ __device__ __forceinline__ void synchrotron_exact(const EmisParams& E, float ln_
 // the FP32 sine and cosine (an orbiting photon's phi grows by 2 pi per turn). Split in two: the fetch
 // computes the cell and issues the loads of its corners (one slice, or the two slices around the
 // emission time); the blend interpolates them. Between the two the tracer computes what does not
-// depend on the density, so the loads' latency is hidden behind that work. Each ray keeps the
-// corners of its last cell (slice k) in registers: consecutive samples of a ray usually fall in the
-// same cell (steps of 0.1-0.4 M against ~1 M cells near the hole), and then the fetch loads nothing,
-// so a ray running alone in its wave does not wait for memory. The values are those in memory, so
-// the result is the same bits either way.
+// depend on the density, so the loads' latency is hidden behind that work. In the persistent engine
+// (CACHE) each ray keeps the corners of its last cell (slice k) in registers: consecutive samples of
+// a ray usually fall in the same cell (steps of 0.1-0.4 M against ~1 M cells near the hole), and
+// then the fetch loads nothing, so a ray running alone in its wave does not wait for memory. The
+// values are those in memory, so the result is the same bits either way.
 struct Corners {
     float c000, c100, c010, c110, c001, c101, c011, c111;
 };
