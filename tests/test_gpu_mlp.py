@@ -144,3 +144,20 @@ def test_surrogate_in_tracer_and_gain(ctx, trained):
     assert 0.97 <= ratio < 1.3   # no gain; the lower bound allows for launch-to-launch spread
     assert infer_ns > 5 * exact_ns
     assert msm < ms64
+
+
+def test_surrogate_tile_records_engines_agree(ctx, trained):
+    """Ray records with the surrogate emissivity: the persistent engine (bh_trace_tile's default,
+    where each ray keeps its turbulence cell and the table's loads overlap the exact formula) and the
+    frame kernel (engine BH_ENGINE_FRAME) give the same bits, on a region across the photon ring."""
+    from test_gpu_batch import assert_records_same
+    t, cfg = ctx
+    t.mlp_bake()
+    F = cfg["F"]
+    for jitter in (0, 1):
+        got = t.trace_tile(F, 448, 448, 128, 128,
+                           bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_MLP, jitter=jitter))
+        want = t.trace_tile(F, 448, 448, 128, 128, bh.Tracer.opts(precision=bh.PREC_MIXED, emissivity=bh.EMIS_MLP,
+                                                                   jitter=jitter, engine=bh.ENGINE_FRAME))
+        assert_records_same(got, want, f"surrogate records jitter {jitter}")
+        assert (got["flow_steps"] > 0).any()
