@@ -1780,13 +1780,15 @@ int bh_vacuum_fates(bh_ctx* c, const double* photons, int n, double tol, double 
     if (n == 0) return BH_OK;
     HIPCK(c, hipSetDevice(c->device));
     if (c->ph_cap < (size_t)n) {
+        c->ph_cap = 0;
         if (c->d_ph) {
             HIPCK(c, hipFree(c->d_ph));
-            HIPCK(c, hipFree(c->d_fates));
+            c->d_ph = nullptr;
         }
-        c->d_ph = nullptr;
-        c->d_fates = nullptr;
-        c->ph_cap = 0;
+        if (c->d_fates) {
+            HIPCK(c, hipFree(c->d_fates));
+            c->d_fates = nullptr;
+        }
         HIPCK(c, hipMalloc((void**)&c->d_ph, sizeof(double) * 9 * n));
         HIPCK(c, hipMalloc((void**)&c->d_fates, sizeof(int) * n));
         c->ph_cap = n;

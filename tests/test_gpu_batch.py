@@ -270,3 +270,12 @@ def test_trace_tile_engines_agree(c2, prec):
         assert_records_same(got, want, f"time slices prec {prec}")
     finally:
         tt.close()
+
+
+def test_trace_tile_errors(c2):
+    """Both engines reject an empty region with the same error."""
+    cfg, t = c2
+    for engine in (bh.ENGINE_AUTO, bh.ENGINE_FRAME):
+        for w, h in ((0, 16), (16, 0)):
+            with pytest.raises(bh.BhError, match="empty tile"):
+                t.trace_tile(cfg["F"], 0, 0, w, h, t.opts(engine=engine))
