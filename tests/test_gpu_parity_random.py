@@ -96,3 +96,33 @@ def test_random_scene_mixed_within_bars(seed):
           f"{smx.step_attempts / smx.samples:.0f} vs {s64.step_attempts / s64.samples:.0f}", flush=True)
     assert flux < 1e-4 and med < 1e-3
     assert smx.step_attempts < s64.step_attempts
+
+
+SHEARED_SMALL = (96, 32, 12, 6)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_scene_sheared_tile_bit_exact(seed, oracle_bhm):
+    """The reference's default turbulence (baked sheared noise, small slices) on random radio scenes:
+    the GPU bakes its slices, the oracle bakes its own, and a 16 x 16 tile across the shadow's edge
+    agrees bit for bit in every per-ray output."""
+    rng = np.random.default_rng(3000 + seed)
+    sets = radio_case(rng).replace("flow.turbulence=0.0", "flow.turbulence=0.3")
+    cfg = setup(shot("c2_radio_1024.ini"), sets, with_grid=False)
+    F = cfg["F"]
+    B = oracle_bhm.sheared_params(cfg["OS"], F.t, F.r, SHEARED_SMALL)
+    oracle_bhm.attach_sheared(cfg["OS"], B, oracle_bhm.sheared_bake(B))
+    t = bh.Tracer(cfg["S"], k2=cfg["k2"], k2_l0=cfg["l0"], k2_dl=cfg["dl"], turb_model=bh.TURB_SHEARED,
+                  ts_dims=SHEARED_SMALL)
+    jitter = bool(rng.integers(0, 2))
+    x0, y0 = (int(v) for v in rng.integers(24, 88, 2))
+    try:
+        gpu = t.trace_tile(F, x0, y0, 16, 16, bh.Tracer.opts(precision=bh.PREC_FP64, jitter=int(jitter)))
+    finally:
+        t.close()
+    px, py = pixel_grid(16, 16, x0, y0)
+    _, ref, _ = oracle_bhm.trace_pixels(cfg["OS"], cfg["OF"], px, py, spp=1, jitter=jitter)
+    print(f"sheared {seed}: {sets} jitter={jitter} tile ({x0}, {y0}): steps {gpu['steps'].min()}..{gpu['steps'].max()}, "
+          f"flow steps {gpu['flow_steps'].sum()}", flush=True)
+    assert_rays_equal(gpu, ref, f"sheared {seed}")
+    assert (gpu["flow_steps"] > 0).any()
