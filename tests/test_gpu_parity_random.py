@@ -151,3 +151,28 @@ def test_random_scene_sheared_mixed_within_bars(seed):
     med = float(np.median(np.abs(Ia[mask] - Ib[mask]) / Ib[mask]))
     print(f"sheared mixed {seed}: flux {flux:.2e} median {med:.2e}", flush=True)
     assert flux < 1e-3 and med < 1e-2
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_random_scene_visible_mixed(seed):
+    """The mixed tracer on random visible (vacuum) scenes, the whole 128 x 128 frame against the FP64
+    tracer: fates and winding counts essentially unchanged, escape directions within a few 1e-6 rad
+    (measured over these 20 scenes: fates identical, windings at least 99.95 %, median |d theta_inf|
+    at most 3.7e-6, p99 at most 1.2e-4), in fewer attempts."""
+    rng = np.random.default_rng(1500 + seed)
+    sets = visible_case(rng)
+    cfg = setup(shot("c1_vacuum_256.ini"), sets)
+    t = tracer_for(cfg)
+    try:
+        a = t.trace_tile(cfg["F"], 0, 0, 128, 128, bh.Tracer.opts(precision=bh.PREC_FP64)).ravel()
+        b = t.trace_tile(cfg["F"], 0, 0, 128, 128, bh.Tracer.opts(precision=bh.PREC_MIXED)).ravel()
+    finally:
+        t.close()
+    esc = (a["fate"] == 1) & (b["fate"] == 1)
+    dth = np.abs(a["th_inf"][esc] - b["th_inf"][esc])
+    print(f"visible mixed {seed}: fates {(a['fate'] == b['fate']).mean():.5f}, windings "
+          f"{(a['ncross'] == b['ncross']).mean():.5f}, |dtheta| median {np.median(dth):.1e}", flush=True)
+    assert (a["fate"] == b["fate"]).mean() > 0.999
+    assert (a["ncross"] == b["ncross"]).mean() > 0.999
+    assert esc.any() and np.median(dth) < 1e-5 and np.quantile(dth, 0.99) < 1e-3
+    assert b["steps"].sum() < a["steps"].sum()
