@@ -1,10 +1,16 @@
-"""The FP64 tracer against the oracle over seeded random scenes, beyond the five named
-configurations: spin, inclination, camera distance, field of view, tolerance, observing frequency,
-flow profile, turbulence amplitude and grid size (radio), sky orientation (visible), pixel-centre or
-jittered samples. Each case traces a 16 x 16 tile at a random position near the centre of a
-128 x 128 frame (across the shadow's edge) and compares every per-ray output bit for bit: the GPU
-and the oracle's bh_math build share the elementary functions, so any difference is a restatement
-or kernel error."""
+"""Seeded random scenes beyond the five named configurations: spin, inclination, camera distance,
+field of view, tolerance, observing frequency, flow profile, turbulence amplitude and grid size
+(radio), sky orientation (visible), pixel-centre or jittered samples.
+
+- The FP64 tracer against the oracle: a 16 x 16 tile at a random position near the centre of a
+  128 x 128 frame (across the shadow's edge), every per-ray output bit for bit (the GPU and the
+  oracle's bh_math build share the elementary functions, so any difference is a restatement or
+  kernel error); the grid turbulence and the reference's sheared turbulence (each side baking its
+  own slices).
+- The mixed tracer against the FP64 tracer on whole frames: radio scenes (grid and sheared
+  turbulence) within flux and pixel-median bars, visible scenes with fates, winding counts and
+  escape directions.
+"""
 import numpy as np
 import pytest
 
